@@ -1,0 +1,192 @@
+/*
+ * sable.h — C-ABI of libsable_amd.so, the MI355X (gfx950) VBR SpMV/SpMM
+ * executor.
+ *
+ * The reference has no in-process FFI for this path: its boundary is
+ * "generate a C program per matrix, compile it with gcc, run it and parse
+ * stdout" (src/codegen.py:293-452 / :827-937, bench_launcher.py:80-85,
+ * eval.py:23-45).  Each entry point below replaces one step of that cycle;
+ * the reference step it stands in for is cited beside it.  All pointers are
+ * plain host or device pointers with explicit sizes; no torch types appear.
+ *
+ * Errors: every function returns SABLE_OK (0) or a nonzero SABLE_ERR_* code
+ * and never exits or aborts; sable_last_error() returns the message of the
+ * calling thread's last failure.
+ *
+ * Threading: a plan is used by one host thread at a time and runs on one HIP
+ * stream (its own, or the one set with sable_plan_set_stream); different
+ * plans may be used from different threads concurrently.
+ */
+#ifndef SABLE_H
+#define SABLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SABLE_OK 0
+#define SABLE_ERR_INVALID 1      /* malformed VBR / bad argument */
+#define SABLE_ERR_HIP 2          /* HIP runtime error */
+#define SABLE_ERR_OOM 3          /* device or host allocation failed */
+#define SABLE_ERR_RTC 4          /* hipRTC compile failure (generic kernel used) */
+#define SABLE_ERR_NO_DEVICE 5    /* no GPU / sable_init not called */
+#define SABLE_ERR_UNSUPPORTED 6  /* option combination not supported */
+
+/* per-block routes (the reference's binary dense / "unrolled" choice,
+ * utils/convert_real_to_vbr.py:52-68, plus the MFMA route for SpMM) */
+#define SABLE_ROUTE_DENSE 0       /* FP32 dense block (GEMV for SpMV, FMA for SpMM) */
+#define SABLE_ROUTE_SPARSE 1      /* nonzeros only, in-block CSR */
+#define SABLE_ROUTE_DENSE_MFMA 2  /* SpMM: bf16 hi+lo split on MFMA (FP32 accumulate) */
+
+#define SABLE_OP_SPMV 0
+#define SABLE_OP_SPMM 1
+
+#define SABLE_MODE_GRID 0         /* one workgroup per tile */
+#define SABLE_MODE_STATIC 1       /* persistent workgroups, static round robin */
+#define SABLE_MODE_QUEUE 2        /* persistent workgroups, atomic work queue */
+
+/* flags for sable_spmv / sable_spmm */
+#define SABLE_HOST_PTRS 1         /* x/y are host pointers: copy in and out */
+#define SABLE_NO_SYNC 2           /* return without synchronising the stream */
+
+typedef struct SablePlan SablePlan;
+typedef struct SableGroup SableGroup;
+
+/* The VBR arrays of src/vbr.py:7-14 (validated: see sable_plan_create). */
+typedef struct {
+    int32_t nbr;              /* block rows = len(rpntr) - 1 */
+    int32_t nbc;              /* block cols = len(cpntr) - 1 */
+    const int32_t *rpntr;     /* nbr + 1 */
+    const int32_t *cpntr;     /* nbc + 1 */
+    const int32_t *bindx;     /* nblocks */
+    const int32_t *bpntrb;    /* nbr, -1 for an empty block row */
+    const int32_t *bpntre;    /* nbr, -1 for an empty block row */
+    const int64_t *indx;      /* nblocks + 1 */
+    const float *val;         /* indx[nblocks] values, each block column-major */
+    int64_t nblocks;
+} SableVbr;
+
+typedef struct {
+    int32_t op;               /* SABLE_OP_SPMV / SABLE_OP_SPMM */
+    int32_t n_rhs;            /* SpMM RHS width N (multiple of 4); 1 for SpMV */
+    int32_t wg;               /* workgroup size: 256 (default) */
+    int32_t xcap;             /* max staged x columns per piece (16..4095) */
+    int64_t tile_bytes;       /* target algorithmic bytes per tile */
+    int32_t max_seg_rows;     /* upper bound on tile height (<= wg) */
+    int32_t min_seg_rows;     /* lower bound on tile height when splitting */
+    int32_t sp_lanes_log2;    /* lanes per sparse row (log2); -1 = per tile */
+    int32_t mode;             /* SABLE_MODE_*; -1 = default (queue) */
+    int32_t grid;             /* persistent grid size; 0 = 8 per CU */
+    int32_t unroll;           /* SpMV dense stream unroll (1,2,4,8); 0 = 4 */
+    int32_t use_rtc;          /* 1 = hipRTC per-plan specialisation (default) */
+    int32_t host_only;        /* 1 = build the host plan only (no device) */
+} SablePlanOpts;
+
+typedef struct {
+    int32_t nrows, ncols;
+    int32_t ntiles, npieces, nxranges;
+    int32_t n_rhs;
+    int64_t nblocks, nblocks_dense, nblocks_sparse, nblocks_mfma;
+    int64_t nnz;              /* stored values != 0 (flops = 2 * nnz * N) */
+    int64_t dense_values;     /* sum over dense-routed blocks of r*c */
+    int64_t dense_padded;     /* dense values as laid out (padding included) */
+    int64_t mfma_values;      /* sum over MFMA-routed blocks of r*c */
+    int64_t mfma_padded;      /* bf16 elements per part, padding included */
+    int64_t sparse_nnz;       /* nonzeros stored in sparse pieces */
+    int64_t rowptr_entries;   /* sparse row pointer entries */
+    int64_t descriptor_bytes; /* tiles + pieces + ranges */
+    int64_t bytes_alg;        /* compulsory bytes of the executed format */
+    int64_t flops_alg;        /* 2 * nnz * N */
+    int64_t device_bytes;     /* device memory held by the plan */
+    int32_t rtc_used;         /* 1 if the launch kernel came from hipRTC */
+    int32_t mode, grid, wg;
+} SablePlanStats;
+
+/* Library. */
+int sable_init(int device);                              /* once per process */
+const char *sable_last_error(void);
+int sable_device_info(int *n_cu, int64_t *hbm_bytes, char *arch, int arch_len);
+int sable_set_cache_dir(const char *dir);                /* hipRTC code-object cache */
+
+/* Replaces: gen_single_threaded_spmv / gen_single_threaded_spmm emitting a
+ * per-matrix C program (src/codegen.py:293-452, :827-937) plus the gcc
+ * compile (bench_launcher.py:85).  Validates the VBR (block rows laid out
+ * consecutively, bindx strictly ascending per row, indx steps equal block
+ * sizes), routes every block by route[k], lays the matrix out in HBM and
+ * selects/compiles the kernel.  Copies all inputs (the caller keeps them). */
+int sable_plan_create(const SableVbr *vbr, const uint8_t *route,
+                      const SablePlanOpts *opts, SablePlan **out);
+int sable_plan_destroy(SablePlan *plan);
+int sable_plan_stats(const SablePlan *plan, SablePlanStats *out);
+int sable_plan_set_stream(SablePlan *plan, void *hip_stream);
+/* Host copies of the plan's arrays (for inspection / tests).  which: 0 tiles,
+ * 1 pieces, 2 xranges, 3 dval, 4 sval, 5 sidx, 6 srowptr, 7 dhi, 8 dlo.
+ * Returns the byte size in *bytes; copies when dst != NULL. */
+int sable_plan_export(const SablePlan *plan, int which, void *dst, int64_t *bytes);
+
+/* Replaces: one execution of the generated program's body,
+ * src/codegen.py:416-437 (memset y; CSR part; every dense block call).
+ * y = A x.  x has ncols floats, y nrows floats (device pointers unless
+ * SABLE_HOST_PTRS). */
+int sable_spmv(SablePlan *plan, const float *x, float *y, int flags);
+
+/* Replaces: the generated SpMM program body, src/codegen.py:895-926 with N
+ * as a parameter.  Y = A X, X [ncols x N] and Y [nrows x N] row-major. */
+int sable_spmm(SablePlan *plan, const float *X, float *Y, int flags);
+
+/* Replaces: the timed loop of src/codegen.py:414-439 and eval.py:23-45.
+ * Runs `warmup` untimed launches then `reps` launches, each bracketed by
+ * HIP events on the plan's stream; times_ms[r] receives each launch's
+ * duration.  When flush_bytes > 0 a buffer of that size is overwritten
+ * before every launch (outside the events) so each launch starts with cold
+ * caches.  x/y (or X/Y) are device pointers. */
+int sable_time(SablePlan *plan, const float *x, float *y, int warmup, int reps,
+               int64_t flush_bytes, float *times_ms);
+
+/* Per-tile wall-clock durations of the last launch made with clocks enabled
+ * (router training labels).  enable=1 turns recording on for later launches;
+ * sable_tile_times copies ntiles durations in nanoseconds. */
+int sable_plan_enable_tile_clock(SablePlan *plan, int enable);
+int sable_tile_times(SablePlan *plan, float *ns, int32_t ntiles);
+
+/* Batching: one persistent launch over the tiles of many plans (each with
+ * its own x/y), captured once into a hipGraph and replayed.  The plans must
+ * share op and n_rhs. */
+int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x,
+                       float *const *y, int mode, int grid, SableGroup **out);
+int sable_group_launch(SableGroup *g, int sync);
+int sable_group_time(SableGroup *g, int warmup, int reps, int64_t flush_bytes,
+                     float *times_ms);
+int sable_group_destroy(SableGroup *g);
+
+/* Router network (2-layer MLP, FP32): forward, backward and Adam kernels.
+ * Weights are device pointers laid out [W1 (H x F), b1 (H), W2 (K x H), b2 (K)]. */
+int sable_mlp_forward(const float *params, int F, int H, int K, const float *X,
+                      int n, float *out, float *hidden);
+int sable_mlp_backward(const float *params, int F, int H, int K, const float *X,
+                       int n, const float *hidden, const float *dout, float *grad);
+/* dout = scale * 2 * (out - target) * mask, elementwise over n values */
+int sable_mlp_mse_grad(const float *out, const float *target, const float *mask, int64_t n,
+                       float scale, float *dout);
+int sable_adam_step(float *params, const float *grad, float *m, float *v, int64_t n,
+                    float lr, float beta1, float beta2, float eps, int step);
+
+/* Build tooling (no reference counterpart): compile one kernel
+ * specialisation into the code-object cache without a device.
+ * kind: 0 SpMV, 1 SpMM FP32, 2 SpMM MFMA. */
+int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int rtmax,
+                         const char *arch);
+
+/* Device memory helpers for callers without another allocator. */
+int sable_malloc(void **ptr, int64_t bytes);
+int sable_free(void *ptr);
+int sable_memcpy_h2d(void *dst, const void *src, int64_t bytes);
+int sable_memcpy_d2h(void *dst, const void *src, int64_t bytes);
+int sable_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SABLE_H */

@@ -1,0 +1,410 @@
+// Host plan builder.  See plan.h.
+//
+// Layout decisions (all per block row, after routing):
+//  * dense-routed blocks of a block row are concatenated in ordinal order:
+//    in the VBR they already form one column-major h x W panel (same height,
+//    consecutive storage), so a row segment of that panel becomes one
+//    contiguous column-major sub-panel per column group;
+//  * sparse-routed blocks keep only their nonzeros (value != 0.0, the
+//    reference's routing count, utils/convert_real_to_vbr.py:41-47) as a CSR
+//    over the segment's rows with uint16 column indices into the group's x
+//    slice (entries of a row in ascending global column order);
+//  * MFMA-routed blocks (SpMM) are split into bf16 hi/lo and stored in the
+//    exact register order of v_mfma_f32_16x16x32_bf16's A operand, so every
+//    lane reads its 8 k-values of hi (and of lo) with one 16-byte load.
+//  * column groups never exceed the LDS staging capacity (xcap columns), and
+//    blocks wider than that are split by columns.
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace sable {
+
+uint16_t f32_to_bf16(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+float bf16_to_f32(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+SablePlanOpts normalize_opts(const SablePlanOpts *in)
+{
+    SablePlanOpts o{};
+    if (in) o = *in;
+    if (o.op != SABLE_OP_SPMM) o.op = SABLE_OP_SPMV;
+    if (o.op == SABLE_OP_SPMV) o.n_rhs = 1;
+    if (o.wg <= 0) o.wg = 256;
+    if (o.xcap <= 0) o.xcap = 4095;
+    if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 48 * 1024 : 64 * 1024;
+    if (o.max_seg_rows <= 0) o.max_seg_rows = o.wg;
+    if (o.min_seg_rows <= 0) o.min_seg_rows = (o.op == SABLE_OP_SPMV) ? 8 : 16;
+    if (o.sp_lanes_log2 > 6) o.sp_lanes_log2 = 6;
+    if (o.sp_lanes_log2 < -1) o.sp_lanes_log2 = -1;
+    if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_QUEUE;
+    if (o.unroll <= 0) o.unroll = 4;
+    return o;
+}
+
+int validate_vbr(const SableVbr &v, std::string &err)
+{
+    if (v.nbr < 1 || v.nbc < 1) { err = "need at least one block row and block column"; return SABLE_ERR_INVALID; }
+    if (!v.rpntr || !v.cpntr || !v.bpntrb || !v.bpntre || !v.indx) { err = "null VBR array"; return SABLE_ERR_INVALID; }
+    if (v.nblocks > 0 && (!v.bindx || !v.val)) { err = "null bindx/val"; return SABLE_ERR_INVALID; }
+    if (v.rpntr[0] != 0 || v.cpntr[0] != 0) { err = "rpntr/cpntr must start at 0"; return SABLE_ERR_INVALID; }
+    for (int a = 0; a < v.nbr; ++a)
+        if (v.rpntr[a + 1] <= v.rpntr[a]) { err = "rpntr must be strictly increasing"; return SABLE_ERR_INVALID; }
+    for (int b = 0; b < v.nbc; ++b)
+        if (v.cpntr[b + 1] <= v.cpntr[b]) { err = "cpntr must be strictly increasing"; return SABLE_ERR_INVALID; }
+    int64_t pos = 0;
+    for (int a = 0; a < v.nbr; ++a) {
+        const int32_t b0 = v.bpntrb[a], b1 = v.bpntre[a];
+        if (b0 == -1 || b1 == -1) {
+            if (b0 != b1) { err = "bpntrb/bpntre must mark empty rows with -1 together"; return SABLE_ERR_INVALID; }
+            continue;
+        }
+        if (b0 != pos || b1 < b0) { err = "block rows must be stored consecutively in row order"; return SABLE_ERR_INVALID; }
+        for (int32_t p = b0; p < b1; ++p) {
+            if (p >= v.nblocks) { err = "bpntre exceeds the number of blocks"; return SABLE_ERR_INVALID; }
+            const int32_t b = v.bindx[p];
+            if (b < 0 || b >= v.nbc) { err = "bindx out of range"; return SABLE_ERR_INVALID; }
+            if (p > b0 && b <= v.bindx[p - 1]) { err = "bindx must be strictly ascending within a block row"; return SABLE_ERR_INVALID; }
+            const int64_t sz = (int64_t)(v.rpntr[a + 1] - v.rpntr[a]) * (v.cpntr[b + 1] - v.cpntr[b]);
+            if (v.indx[p + 1] - v.indx[p] != sz) { err = "indx steps must equal block height x width"; return SABLE_ERR_INVALID; }
+        }
+        pos = b1;
+    }
+    if (pos != v.nblocks) { err = "block count of the rows differs from nblocks"; return SABLE_ERR_INVALID; }
+    if (v.indx[0] != 0) { err = "indx must start at 0"; return SABLE_ERR_INVALID; }
+    return SABLE_OK;
+}
+
+namespace {
+
+struct ColSeg {       // columns [cb, cb+n) of block k
+    int64_t k;
+    int32_t cb, n;
+    int32_t gcol;     // global column of cb
+};
+
+// Splits the blocks of one route into column groups of at most cap columns.
+std::vector<std::vector<ColSeg>> group_columns(const std::vector<int64_t> &blocks,
+                                               const SableVbr &v, int cap)
+{
+    std::vector<std::vector<ColSeg>> groups;
+    std::vector<ColSeg> cur;
+    int width = 0;
+    for (int64_t k : blocks) {
+        const int32_t b = v.bindx[k];
+        const int32_t w = v.cpntr[b + 1] - v.cpntr[b];
+        int32_t cb = 0;
+        while (cb < w) {
+            if (width == cap) { groups.push_back(cur); cur.clear(); width = 0; }
+            const int32_t n = std::min(w - cb, cap - width);
+            cur.push_back({k, cb, n, v.cpntr[b] + cb});
+            width += n;
+            cb += n;
+        }
+    }
+    if (!cur.empty()) groups.push_back(cur);
+    return groups;
+}
+
+int group_width(const std::vector<ColSeg> &g)
+{
+    int w = 0;
+    for (auto &s : g) w += s.n;
+    return w;
+}
+
+void push_xranges(HostPlan &P, const std::vector<ColSeg> &g, SbPiece &pc)
+{
+    pc.xr_begin = (int32_t)P.xr.size();
+    for (auto &s : g) {
+        if ((int32_t)P.xr.size() > pc.xr_begin) {
+            SbXRange &last = P.xr.back();
+            if (last.col0 + last.len == s.gcol) { last.len += s.n; continue; }
+        }
+        P.xr.push_back({s.gcol, s.n});
+    }
+    pc.xr_end = (int32_t)P.xr.size();
+}
+
+int ilog2_ceil(int64_t v)
+{
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+
+void dense_stream_params(int hs, int wg, int32_t &tpar, int32_t &mcols)
+{
+    // 4*T' must be a multiple of lcm(hs, 4) and T' <= wg.
+    const int l = std::lcm(hs, 4);
+    const int groups = (4 * wg) / l;
+    tpar = groups * l / 4;
+    mcols = 4 * tpar / hs;
+}
+
+}  // namespace
+
+int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o,
+                    HostPlan &P, std::string &err)
+{
+    int rc = validate_vbr(v, err);
+    if (rc) return rc;
+    if (o.wg != 64 && o.wg != 128 && o.wg != 256 && o.wg != 512) { err = "wg must be 64, 128, 256 or 512"; return SABLE_ERR_INVALID; }
+    if (o.xcap < 16 || o.xcap > 4095) { err = "xcap must be in [16, 4095]"; return SABLE_ERR_INVALID; }
+    const bool spmm = o.op == SABLE_OP_SPMM;
+    if (spmm && (o.n_rhs < 4 || o.n_rhs % 4 != 0 || o.n_rhs / 4 > o.wg)) {
+        err = "SpMM needs n_rhs a positive multiple of 4 with n_rhs/4 <= wg";
+        return SABLE_ERR_UNSUPPORTED;
+    }
+    P = HostPlan();
+    P.op = o.op;
+    P.n_rhs = spmm ? o.n_rhs : 1;
+    P.nrows = v.rpntr[v.nbr];
+    P.ncols = v.cpntr[v.nbc];
+    P.wg = o.wg;
+    SablePlanStats &S = P.stats;
+    S.nrows = P.nrows;
+    S.ncols = P.ncols;
+    S.n_rhs = P.n_rhs;
+    S.nblocks = v.nblocks;
+
+    // rows per tile bound: SpMV needs hs <= wg (flat stream lanes); the FP32
+    // SpMM kernel needs hs <= (wg / (N/4)) * 8 register rows.
+    int max_rows = std::min(o.max_seg_rows, o.wg);
+    // SpMM: the FP32 kernel holds at most 8 rows per thread group
+    // ((wg / (N/4)) * 8 rows) and the MFMA kernel at most 64 rows per tile.
+    if (spmm) max_rows = std::min({max_rows, (o.wg / (o.n_rhs / 4)) * 8, 64});
+    max_rows = std::max(max_rows, 1);
+    const int min_rows = std::max(1, std::min(o.min_seg_rows, max_rows));
+
+    // count nonzeros (plan-invariant flops basis)
+    for (int64_t i = 0; i < v.indx[v.nblocks]; ++i) S.nnz += (v.val[i] != 0.0f);
+
+    for (int32_t a = 0; a < v.nbr; ++a) {
+        const int32_t r0 = v.rpntr[a], h = v.rpntr[a + 1] - r0;
+        std::vector<int64_t> dense, sparse, mfma;
+        if (v.bpntrb[a] != -1) {
+            for (int64_t k = v.bpntrb[a]; k < v.bpntre[a]; ++k) {
+                const uint8_t rt = route ? route[k] : (uint8_t)SABLE_ROUTE_DENSE;
+                if (rt == SABLE_ROUTE_SPARSE) sparse.push_back(k);
+                else if (rt == SABLE_ROUTE_DENSE_MFMA && spmm) {
+                    if (o.n_rhs % 16 != 0 || o.n_rhs > 256) {
+                        err = "the MFMA route needs n_rhs a multiple of 16 and at most 256";
+                        return SABLE_ERR_UNSUPPORTED;
+                    }
+                    mfma.push_back(k);
+                }
+                else if (rt == SABLE_ROUTE_DENSE || rt == SABLE_ROUTE_DENSE_MFMA) dense.push_back(k);
+                else { err = "unknown route value"; return SABLE_ERR_INVALID; }
+            }
+        }
+        S.nblocks_dense += (int64_t)dense.size();
+        S.nblocks_sparse += (int64_t)sparse.size();
+        S.nblocks_mfma += (int64_t)mfma.size();
+        auto dgroups = group_columns(dense, v, o.xcap);
+        auto sgroups = group_columns(sparse, v, o.xcap);
+        auto mgroups = group_columns(mfma, v, o.xcap);
+
+        // sparse nonzeros per row of the block row (for costing / lanes)
+        std::vector<int32_t> row_nnz(h, 0);
+        int64_t sp_total = 0;
+        for (int64_t k : sparse) {
+            const float *bv = v.val + v.indx[k];
+            const int32_t w = v.cpntr[v.bindx[k] + 1] - v.cpntr[v.bindx[k]];
+            for (int32_t c = 0; c < w; ++c)
+                for (int32_t r = 0; r < h; ++r)
+                    if (bv[(int64_t)c * h + r] != 0.0f) { ++row_nnz[r]; ++sp_total; }
+        }
+        int64_t wd = 0, wm = 0;
+        for (auto &g : dgroups) wd += group_width(g);
+        for (auto &g : mgroups) wm += group_width(g);
+        const double work = 4.0 * h * (double)(wd + wm) + 6.0 * sp_total + 4.0 * (h + 1) * sgroups.size();
+
+        // row segmentation
+        int64_t nseg = (h + max_rows - 1) / max_rows;
+        const int64_t want = (int64_t)std::ceil(work / (double)o.tile_bytes);
+        nseg = std::max<int64_t>(nseg, std::min<int64_t>(want, (h + min_rows - 1) / min_rows));
+        nseg = std::max<int64_t>(nseg, 1);
+        std::vector<int32_t> seg_start;
+        if (spmm && !mgroups.empty()) {
+            // MFMA tiles: heights in multiples of 16 rows where possible
+            const int64_t m16 = (h + 15) / 16;
+            nseg = std::min<int64_t>(nseg, m16);
+            for (int64_t s = 0; s < nseg; ++s) seg_start.push_back((int32_t)(16 * ((m16 * s) / nseg)));
+        } else {
+            for (int64_t s = 0; s < nseg; ++s) seg_start.push_back((int32_t)((h * s) / nseg));
+        }
+        seg_start.push_back(h);
+
+        for (size_t s = 0; s + 1 < seg_start.size(); ++s) {
+            const int32_t s0 = seg_start[s], hs = seg_start[s + 1] - s0;
+            if (hs <= 0) continue;
+            SbTile t{};
+            t.row0 = r0 + s0;
+            t.nrows = hs;
+            t.mat = 0;
+            dense_stream_params(std::min<int32_t>(hs, o.wg), o.wg, t.dense_tpar, t.dense_mcols);
+            t.piece_begin = (int32_t)P.pieces.size();
+            double cost = 64.0;
+            // dense FP32 pieces
+            for (auto &g : dgroups) {
+                SbPiece pc{};
+                pc.kind = SB_PIECE_DENSE;
+                pc.ncols = group_width(g);
+                while (P.dval.size() % 4) P.dval.push_back(0.f);
+                pc.data_off = (int64_t)P.dval.size();
+                for (auto &cs : g) {
+                    const float *bv = v.val + v.indx[cs.k];
+                    for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c) {
+                        const float *col = bv + (int64_t)c * h + s0;
+                        P.dval.insert(P.dval.end(), col, col + hs);
+                    }
+                }
+                while (P.dval.size() % 4) P.dval.push_back(0.f);
+                S.dense_values += (int64_t)hs * pc.ncols;
+                push_xranges(P, g, pc);
+                P.pieces.push_back(pc);
+                cost += 4.0 * hs * pc.ncols;
+            }
+            // MFMA bf16 hi/lo pieces (SpMM)
+            for (auto &g : mgroups) {
+                SbPiece pc{};
+                pc.kind = SB_PIECE_DENSE_BF16;
+                pc.ncols = group_width(g);
+                pc.ksteps = (pc.ncols + 31) / 32;
+                const int32_t mt = (hs + 15) / 16;
+                pc.data_off = (int64_t)P.dhi.size();
+                const int64_t n = (int64_t)mt * pc.ksteps * 64 * 8;
+                P.dhi.resize(P.dhi.size() + n, 0);
+                P.dlo.resize(P.dlo.size() + n, 0);
+                uint16_t *hi = P.dhi.data() + pc.data_off;
+                uint16_t *lo = P.dlo.data() + pc.data_off;
+                int32_t c = 0;
+                for (auto &cs : g) {
+                    const float *bv = v.val + v.indx[cs.k];
+                    for (int32_t cc = cs.cb; cc < cs.cb + cs.n; ++cc, ++c) {
+                        const float *col = bv + (int64_t)cc * h + s0;
+                        const int32_t ks = c / 32, kk = c % 32;
+                        for (int32_t r = 0; r < hs; ++r) {
+                            const int32_t lane = (r % 16) + 16 * (kk / 8);
+                            const int64_t off = (((int64_t)(r / 16) * pc.ksteps + ks) * 64 + lane) * 8 + (kk % 8);
+                            const float f = col[r];
+                            const uint16_t bh = f32_to_bf16(f);
+                            hi[off] = bh;
+                            lo[off] = f32_to_bf16(f - bf16_to_f32(bh));
+                        }
+                    }
+                }
+                S.mfma_values += (int64_t)hs * pc.ncols;
+                S.mfma_padded += n;
+                push_xranges(P, g, pc);
+                P.pieces.push_back(pc);
+                cost += 4.0 * hs * pc.ncols;
+            }
+            // sparse pieces
+            int64_t seg_sp = 0;
+            for (auto &g : sgroups) {
+                // count
+                std::vector<int32_t> cnt(hs, 0);
+                for (auto &cs : g) {
+                    const float *bv = v.val + v.indx[cs.k];
+                    for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c) {
+                        const float *col = bv + (int64_t)c * h + s0;
+                        for (int32_t r = 0; r < hs; ++r) cnt[r] += (col[r] != 0.0f);
+                    }
+                }
+                int64_t tot = 0;
+                for (int32_t r = 0; r < hs; ++r) tot += cnt[r];
+                if (tot == 0) continue;   // no nonzeros in these rows: contributes nothing
+                SbPiece pc{};
+                pc.kind = SB_PIECE_SPARSE;
+                pc.ncols = group_width(g);
+                pc.data_off = (int64_t)P.sval.size();
+                pc.rowptr_off = (int32_t)P.srowptr.size();
+                std::vector<int32_t> cur(hs);
+                int32_t acc = 0;
+                for (int32_t r = 0; r < hs; ++r) {
+                    P.srowptr.push_back(acc);
+                    cur[r] = acc;
+                    acc += cnt[r];
+                }
+                P.srowptr.push_back(acc);
+                P.sval.resize(P.sval.size() + tot);
+                P.sidx.resize(P.sidx.size() + tot);
+                float *sv = P.sval.data() + pc.data_off;
+                uint16_t *si = P.sidx.data() + pc.data_off;
+                int32_t lc = 0;
+                for (auto &cs : g) {
+                    const float *bv = v.val + v.indx[cs.k];
+                    for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c, ++lc) {
+                        const float *col = bv + (int64_t)c * h + s0;
+                        for (int32_t r = 0; r < hs; ++r) {
+                            if (col[r] != 0.0f) {
+                                sv[cur[r]] = col[r];
+                                si[cur[r]] = (uint16_t)lc;
+                                ++cur[r];
+                            }
+                        }
+                    }
+                }
+                S.sparse_nnz += tot;
+                S.rowptr_entries += hs + 1;
+                seg_sp += tot;
+                push_xranges(P, g, pc);
+                P.pieces.push_back(pc);
+                cost += 6.0 * tot + 4.0 * (hs + 1);
+            }
+            t.piece_end = (int32_t)P.pieces.size();
+            const int64_t avg = (seg_sp + hs - 1) / std::max(hs, 1);
+            int lg = o.sp_lanes_log2 >= 0 ? o.sp_lanes_log2 : ilog2_ceil(std::max<int64_t>(1, avg / 2));
+            t.sp_lanes_log2 = std::min(lg, 6);
+            if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
+            P.tiles.push_back(t);
+            P.tile_cost.push_back(cost);
+        }
+    }
+    // largest tiles first (LPT order for the persistent queue)
+    std::vector<int32_t> order(P.tiles.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return P.tile_cost[x] > P.tile_cost[y]; });
+    std::vector<SbTile> tiles(P.tiles.size());
+    std::vector<double> cost(P.tiles.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        tiles[i] = P.tiles[order[i]];
+        cost[i] = P.tile_cost[order[i]];
+    }
+    P.tiles.swap(tiles);
+    P.tile_cost.swap(cost);
+
+    S.ntiles = (int32_t)P.tiles.size();
+    S.npieces = (int32_t)P.pieces.size();
+    S.nxranges = (int32_t)P.xr.size();
+    S.dense_padded = (int64_t)P.dval.size();
+    S.descriptor_bytes = (int64_t)(P.tiles.size() * sizeof(SbTile) + P.pieces.size() * sizeof(SbPiece) +
+                                   P.xr.size() * sizeof(SbXRange));
+    S.bytes_alg = 4 * S.dense_values + 4 * S.mfma_values + 6 * S.sparse_nnz + 4 * S.rowptr_entries +
+                  S.descriptor_bytes + 4LL * P.ncols * P.n_rhs + 4LL * P.nrows * P.n_rhs;
+    S.flops_alg = 2 * S.nnz * (int64_t)P.n_rhs;
+    S.wg = o.wg;
+    S.mode = o.mode;
+    return SABLE_OK;
+}
+
+}  // namespace sable

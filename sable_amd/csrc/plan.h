@@ -1,0 +1,45 @@
+// Host-side plan builder: VBR + per-block routes -> HBM layout + tile list.
+// Plain C++17 (no HIP), so it is built and tested on machines without a GPU.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sable_device.h"
+#include "../../include/sable.h"
+
+namespace sable {
+
+struct HostPlan {
+    int32_t op = SABLE_OP_SPMV;
+    int32_t n_rhs = 1;
+    int32_t nrows = 0, ncols = 0;
+    int32_t wg = 256;
+    std::vector<SbTile> tiles;
+    std::vector<SbPiece> pieces;
+    std::vector<SbXRange> xr;
+    std::vector<float> dval;      // FP32 dense pieces
+    std::vector<uint16_t> dhi;    // bf16 hi (MFMA order)
+    std::vector<uint16_t> dlo;    // bf16 lo (MFMA order)
+    std::vector<float> sval;
+    std::vector<uint16_t> sidx;
+    std::vector<int32_t> srowptr;
+    std::vector<double> tile_cost;   // estimated bytes per tile (ordering)
+    SablePlanStats stats{};
+};
+
+// Fills defaults for zero / negative fields.
+SablePlanOpts normalize_opts(const SablePlanOpts *in);
+
+// Returns SABLE_OK or an error code with a message in err.
+int validate_vbr(const SableVbr &v, std::string &err);
+
+// Builds the host plan.  route may be null (all dense).
+int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o,
+                    HostPlan &out, std::string &err);
+
+// bf16 helpers (round to nearest even; NaN kept quiet)
+uint16_t f32_to_bf16(float f);
+float bf16_to_f32(uint16_t h);
+
+}  // namespace sable

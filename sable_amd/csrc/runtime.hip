@@ -1,0 +1,996 @@
+// libsable_amd.so: C-ABI runtime of the VBR SpMV/SpMM executor on gfx950.
+//
+// Responsibilities: device selection, plan upload into HBM, per-plan kernel
+// specialisation through hipRTC (with an on-disk code-object cache and the
+// ahead-of-time generic kernels as fallback), launches in three scheduling
+// modes, HIP-event timing with optional cache flushing, per-tile clocks,
+// graph-captured multi-plan launches, and the router MLP kernels.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "../../include/sable.h"
+#include "plan.h"
+#include "sable_kernels.h"
+#include "sable_mfma.h"
+#include "sable_mlp.h"
+
+// ---------------------------------------------------------------------------
+// ahead-of-time generic instances (fallback when hipRTC is unavailable)
+
+extern "C" __global__ void __launch_bounds__(256) sb_spmv_aot(SbSpmvArgs a)
+{
+    sb_spmv_body<256, 4, 4096>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(256) sb_spmm_f32_aot(SbSpmmArgs a)
+{
+    sb_spmm_f32_body<256, 8, 0, 4096>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(256) sb_spmm_mfma_aot(SbSpmmArgs a)
+{
+    sb_spmm_mfma_body<256, 0, 4096>(a);
+}
+
+extern "C" __global__ void sb_fill_u32(uint32_t *p, int64_t n, uint32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// embedded kernel sources for hipRTC (generated at build time)
+#include "_embedded_src.inc"
+
+namespace {
+
+thread_local std::string g_err;
+std::mutex g_mu;
+int g_device = -1;
+int g_ncu = 0;
+int64_t g_hbm = 0;
+std::string g_arch = "gfx950";
+std::string g_cache_dir;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPOK(expr)                                                                           \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            return fail(_e == hipErrorOutOfMemory ? SABLE_ERR_OOM : SABLE_ERR_HIP,            \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+        }                                                                                     \
+    } while (0)
+
+uint64_t fnv1a(const std::string &s)
+{
+    uint64_t h = 1469598103934665603ULL;
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+struct RtcKernel {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+};
+std::map<std::string, RtcKernel> g_rtc;   // key -> loaded kernel
+
+std::string cache_dir()
+{
+    if (!g_cache_dir.empty()) return g_cache_dir;
+    const char *e = getenv("SABLE_CACHE_DIR");
+    if (e && *e) return e;
+    return "/tmp/sable_amd_rtc";
+}
+
+void mkdirs(const std::string &d)
+{
+    std::string cur;
+    std::stringstream ss(d);
+    std::string part;
+    if (!d.empty() && d[0] == '/') cur = "/";
+    while (std::getline(ss, part, '/')) {
+        if (part.empty()) continue;
+        cur += part + "/";
+        mkdir(cur.c_str(), 0755);
+    }
+}
+
+// Compiles `entry` (a kernel defined in the generated stub) with the given
+// macro definitions; returns the code object.  No device needed.
+int rtc_compile(const std::string &defines, const std::string &stub, std::vector<char> &code,
+                std::string &log)
+{
+    const std::string src = defines + "#include \"sable_kernels.h\"\n#include \"sable_mfma.h\"\n" + stub;
+    const char *hdrs[] = {k_src_sable_device_h, k_src_sable_kernels_h, k_src_sable_mfma_h};
+    const char *names[] = {"sable_device.h", "sable_kernels.h", "sable_mfma.h"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "sable_rtc.hip", 3, hdrs, names) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return SABLE_ERR_RTC;
+    }
+    const std::string arch = "--offload-arch=" + g_arch;
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    log.assign(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return SABLE_ERR_RTC;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return SABLE_OK;
+}
+
+// Returns (compiling and caching if needed) the code object for a key.
+int rtc_code_object(const std::string &defines, const std::string &stub, std::vector<char> &code,
+                    std::string &path_out)
+{
+    const std::string key = g_arch + "\n" + defines + stub + k_src_sable_kernels_h + k_src_sable_mfma_h +
+                            k_src_sable_device_h;
+    char name[64];
+    snprintf(name, sizeof name, "sb_%016llx.co", (unsigned long long)fnv1a(key));
+    const std::string dir = cache_dir();
+    const std::string path = dir + "/" + name;
+    path_out = path;
+    {
+        std::ifstream f(path, std::ios::binary);
+        if (f) {
+            code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+            if (!code.empty()) return SABLE_OK;
+        }
+    }
+    std::string log;
+    int rc = rtc_compile(defines, stub, code, log);
+    if (rc) return fail(rc, "hipRTC compile failed: " + log.substr(0, 2000));
+    mkdirs(dir);
+    const std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream o(tmp, std::ios::binary);
+        o.write(code.data(), (std::streamsize)code.size());
+    }
+    rename(tmp.c_str(), path.c_str());
+    return SABLE_OK;
+}
+
+int rtc_get(const std::string &defines, const std::string &stub, const char *entry, hipFunction_t *fn)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    const std::string key = defines + stub;
+    auto it = g_rtc.find(key);
+    if (it != g_rtc.end()) {
+        *fn = it->second.fn;
+        return SABLE_OK;
+    }
+    std::vector<char> code;
+    std::string path;
+    int rc = rtc_code_object(defines, stub, code, path);
+    if (rc) return rc;
+    RtcKernel k;
+    HIPOK(hipModuleLoadData(&k.mod, code.data()));
+    HIPOK(hipModuleGetFunction(&k.fn, k.mod, entry));
+    g_rtc[key] = k;
+    *fn = k.fn;
+    return SABLE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// specialisations
+
+struct Spec {
+    int wg = 256, unroll = 4, xcap = 4096, nrhs = 0, rtmax = 8;
+};
+
+std::string spec_defines(const Spec &s)
+{
+    char b[256];
+    snprintf(b, sizeof b, "#define SB_WG %d\n#define SB_UNROLL %d\n#define SB_XCAP %d\n#define SB_NRHS %d\n#define SB_RTMAX %d\n",
+             s.wg, s.unroll, s.xcap, s.nrhs, s.rtmax);
+    return b;
+}
+
+const char *kStubSpmv =
+    "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmv_rtc(SbSpmvArgs a)\n"
+    "{ sb_spmv_body<SB_WG, SB_UNROLL, SB_XCAP>(a); }\n";
+const char *kStubSpmmF32 =
+    "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_f32_rtc(SbSpmmArgs a)\n"
+    "{ sb_spmm_f32_body<SB_WG, SB_RTMAX, SB_NRHS, SB_XCAP>(a); }\n";
+const char *kStubSpmmMfma =
+    "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_mfma_rtc(SbSpmmArgs a)\n"
+    "{ sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP>(a); }\n";
+
+int pow2_at_least(int v)
+{
+    int p = 16;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan object
+
+struct SablePlan {
+    sable::HostPlan host;
+    bool host_only = false;
+    bool keep_host = true;
+    SablePlanOpts opts{};
+    SablePlanStats stats{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // device arrays
+    SbTile *d_tiles = nullptr;
+    SbPiece *d_pieces = nullptr;
+    SbXRange *d_xr = nullptr;
+    float *d_dval = nullptr;
+    uint16_t *d_dhi = nullptr, *d_dlo = nullptr;
+    float *d_sval = nullptr;
+    uint16_t *d_sidx = nullptr;
+    int32_t *d_srowptr = nullptr;
+    int32_t *d_queue = nullptr;
+    const float **d_xptr = nullptr;
+    float **d_yptr = nullptr;
+    uint64_t *d_clock = nullptr;
+    bool clock_on = false;
+    // host staging for SABLE_HOST_PTRS
+    float *d_xtmp = nullptr, *d_ytmp = nullptr;
+    int64_t xtmp_n = 0, ytmp_n = 0;
+    // flush buffer for cold timing
+    uint32_t *d_flush = nullptr;
+    int64_t flush_n = 0;
+    // kernel
+    bool rtc = false;
+    hipFunction_t fn = nullptr;
+    const void *aot = nullptr;
+    int grid = 0;
+    int64_t device_bytes = 0;
+};
+
+struct SableGroup {
+    std::vector<SablePlan *> plans;
+    hipStream_t stream = nullptr;
+    SbTile *d_tiles = nullptr;
+    SbPiece *d_pieces = nullptr;
+    SbXRange *d_xr = nullptr;
+    float *d_dval = nullptr;
+    uint16_t *d_dhi = nullptr, *d_dlo = nullptr;
+    float *d_sval = nullptr;
+    uint16_t *d_sidx = nullptr;
+    int32_t *d_srowptr = nullptr;
+    int32_t *d_queue = nullptr;
+    const float **d_xptr = nullptr;
+    float **d_yptr = nullptr;
+    int32_t ntiles = 0;
+    int32_t op = 0, n_rhs = 1, mode = SABLE_MODE_QUEUE, grid = 0;
+    hipFunction_t fn = nullptr;
+    const void *aot = nullptr;
+    bool rtc = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t *d_flush = nullptr;
+    int64_t flush_n = 0;
+};
+
+namespace {
+
+template <typename T>
+int upload(T **dptr, const std::vector<T> &v, int64_t &bytes)
+{
+    *dptr = nullptr;
+    if (v.empty()) return SABLE_OK;
+    HIPOK(hipMalloc((void **)dptr, v.size() * sizeof(T)));
+    HIPOK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    bytes += (int64_t)(v.size() * sizeof(T));
+    return SABLE_OK;
+}
+
+#define CHECKRC(x)              \
+    do {                        \
+        int _rc = (x);          \
+        if (_rc) return _rc;    \
+    } while (0)
+
+int max_piece_cols(const sable::HostPlan &h)
+{
+    int m = 1;
+    for (auto &p : h.pieces) m = std::max(m, p.ncols);
+    return m;
+}
+
+bool has_kind(const sable::HostPlan &h, int kind)
+{
+    for (auto &p : h.pieces)
+        if (p.kind == kind) return true;
+    return false;
+}
+
+// chooses the kernel for a plan (RTC specialisation or AOT generic)
+int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, hipFunction_t &fn,
+                  const void *&aot)
+{
+    rtc = false;
+    fn = nullptr;
+    Spec s;
+    s.wg = o.wg;
+    s.unroll = o.unroll;
+    s.xcap = std::min(4096, pow2_at_least(max_piece_cols(h) + 1));
+    const bool spmm = h.op == SABLE_OP_SPMM;
+    const bool mfma = spmm && has_kind(h, SB_PIECE_DENSE_BF16);
+    if (spmm) {
+        s.nrhs = h.n_rhs;
+        int rows = 1;
+        for (auto &t : h.tiles) rows = std::max(rows, t.nrows);
+        const int ngroups = std::max(1, o.wg / (h.n_rhs / 4));
+        s.rtmax = std::max(1, (rows + ngroups - 1) / ngroups);
+    }
+    if (spmm) aot = mfma ? (const void *)sb_spmm_mfma_aot : (const void *)sb_spmm_f32_aot;
+    else aot = (const void *)sb_spmv_aot;
+    if (!o.use_rtc) {
+        if (o.wg != 256 || (!spmm && o.unroll != 4)) return fail(SABLE_ERR_UNSUPPORTED, "AOT kernels exist only for wg=256 / unroll=4");
+        return SABLE_OK;
+    }
+    const char *stub = spmm ? (mfma ? kStubSpmmMfma : kStubSpmmF32) : kStubSpmv;
+    const char *entry = spmm ? (mfma ? "sb_spmm_mfma_rtc" : "sb_spmm_f32_rtc") : "sb_spmv_rtc";
+    int rc = rtc_get(spec_defines(s), stub, entry, &fn);
+    if (rc == SABLE_OK) {
+        rtc = true;
+        return SABLE_OK;
+    }
+    // fall back to the generic kernel; keep the message for the caller
+    if (o.wg != 256 || (!spmm && o.unroll != 4)) return rc;
+    fprintf(stderr, "sable: %s; using the generic kernel\n", g_err.c_str());
+    return SABLE_OK;
+}
+
+int occupancy_grid(bool rtc, hipFunction_t fn, const void *aot, int wg, int ntiles, int req)
+{
+    if (req > 0) return std::max(1, std::min(req, ntiles));
+    int per_cu = 0;
+    if (rtc) {
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, wg, 0) != hipSuccess) per_cu = 0;
+    } else {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, aot, wg, 0) != hipSuccess) per_cu = 0;
+    }
+    if (per_cu <= 0) per_cu = 4;
+    int64_t g = (int64_t)per_cu * std::max(g_ncu, 1);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
+}
+
+int launch_raw(bool rtc, hipFunction_t fn, const void *aot, int grid, int wg, void *argp,
+               hipStream_t st)
+{
+    void *args[] = {argp};
+    if (rtc) {
+        HIPOK(hipModuleLaunchKernel(fn, grid, 1, 1, wg, 1, 1, 0, st, args, nullptr));
+    } else {
+        HIPOK(hipLaunchKernel(aot, dim3(grid), dim3(wg), args, 0, st));
+    }
+    return SABLE_OK;
+}
+
+int plan_launch(SablePlan *p, const float *x, float *y)
+{
+    const sable::HostPlan &h = p->host;
+    const int ntiles = p->stats.ntiles;
+    if (ntiles == 0) return SABLE_OK;
+    const int mode = p->opts.mode;
+    const int grid = mode == SABLE_MODE_GRID ? ntiles : p->grid;
+    if (h.op == SABLE_OP_SPMV) {
+        SbSpmvArgs a{};
+        a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
+        a.dval = p->d_dval; a.sval = p->d_sval; a.sidx = p->d_sidx; a.srowptr = p->d_srowptr;
+        a.xvec = nullptr; a.yvec = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
+        a.tile_clock = p->clock_on ? p->d_clock : nullptr;
+        a.ntiles = ntiles; a.mode = mode;
+        return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream);
+    }
+    SbSpmmArgs a{};
+    a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
+    a.dval = p->d_dval; a.dhi = p->d_dhi; a.dlo = p->d_dlo;
+    a.sval = p->d_sval; a.sidx = p->d_sidx; a.srowptr = p->d_srowptr;
+    a.xmat = nullptr; a.ymat = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
+    a.tile_clock = p->clock_on ? p->d_clock : nullptr;
+    a.ntiles = ntiles; a.mode = mode; a.n_rhs = h.n_rhs;
+    return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream);
+}
+
+int ensure_flush(uint32_t **buf, int64_t *n, int64_t bytes)
+{
+    const int64_t want = (bytes + 3) / 4;
+    if (*n >= want) return SABLE_OK;
+    if (*buf) hipFree(*buf);
+    *buf = nullptr;
+    *n = 0;
+    HIPOK(hipMalloc((void **)buf, want * 4));
+    *n = want;
+    return SABLE_OK;
+}
+
+int flush_caches(uint32_t *buf, int64_t n, hipStream_t st, uint32_t salt)
+{
+    if (!buf || n == 0) return SABLE_OK;
+    hipLaunchKernelGGL(sb_fill_u32, dim3(std::max(1, g_ncu) * 8), dim3(256), 0, st, buf, n, salt);
+    HIPOK(hipGetLastError());
+    return SABLE_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+
+extern "C" {
+
+const char *sable_last_error(void) { return g_err.c_str(); }
+
+int sable_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(SABLE_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(SABLE_ERR_INVALID, "device index out of range");
+    HIPOK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPOK(hipGetDeviceProperties(&prop, device));
+    g_device = device;
+    g_ncu = prop.multiProcessorCount;
+    g_hbm = (int64_t)prop.totalGlobalMem;
+    std::string arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch = arch.substr(0, colon);
+    if (arch.rfind("gfx950", 0) != 0)
+        fprintf(stderr, "sable: device is %s; kernels are written for gfx950\n", arch.c_str());
+    g_arch = arch;
+    return SABLE_OK;
+}
+
+int sable_device_info(int *n_cu, int64_t *hbm_bytes, char *arch, int arch_len)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (n_cu) *n_cu = g_ncu;
+    if (hbm_bytes) *hbm_bytes = g_hbm;
+    if (arch && arch_len > 0) {
+        strncpy(arch, g_arch.c_str(), arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return SABLE_OK;
+}
+
+int sable_set_cache_dir(const char *dir)
+{
+    g_cache_dir = dir ? dir : "";
+    return SABLE_OK;
+}
+
+// Compiles (and caches) one kernel specialisation without a device: used at
+// build time so the code objects ship with the library.  No counterpart in
+// the reference (build tooling).  kind: 0 spmv, 1 spmm f32, 2 spmm mfma.
+int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int rtmax,
+                         const char *arch)
+{
+    if (arch && *arch) g_arch = arch;
+    Spec s;
+    s.wg = wg; s.unroll = unroll; s.xcap = xcap; s.nrhs = nrhs; s.rtmax = rtmax;
+    const char *stub = kind == 0 ? kStubSpmv : (kind == 1 ? kStubSpmmF32 : kStubSpmmMfma);
+    std::vector<char> code;
+    std::string path;
+    return rtc_code_object(spec_defines(s), stub, code, path);
+}
+
+int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlanOpts *opts_in,
+                      SablePlan **out)
+{
+    if (!vbr || !out) return fail(SABLE_ERR_INVALID, "null argument");
+    *out = nullptr;
+    SablePlanOpts o = sable::normalize_opts(opts_in);
+    std::unique_ptr<SablePlan> p(new SablePlan());
+    p->opts = o;
+    std::string err;
+    int rc;
+    try {
+        rc = sable::build_host_plan(*vbr, route, o, p->host, err);
+    } catch (const std::bad_alloc &) {
+        return fail(SABLE_ERR_OOM, "host allocation failed while building the plan");
+    }
+    if (rc) return fail(rc, err);
+    p->stats = p->host.stats;
+    p->host_only = o.host_only != 0;
+    if (p->host_only) {
+        *out = p.release();
+        return SABLE_OK;
+    }
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    CHECKRC(select_kernel(p->host, o, p->rtc, p->fn, p->aot));
+    const sable::HostPlan &h = p->host;
+    int64_t bytes = 0;
+    CHECKRC(upload(&p->d_tiles, h.tiles, bytes));
+    CHECKRC(upload(&p->d_pieces, h.pieces, bytes));
+    CHECKRC(upload(&p->d_xr, h.xr, bytes));
+    CHECKRC(upload(&p->d_dval, h.dval, bytes));
+    CHECKRC(upload(&p->d_dhi, h.dhi, bytes));
+    CHECKRC(upload(&p->d_dlo, h.dlo, bytes));
+    CHECKRC(upload(&p->d_sval, h.sval, bytes));
+    CHECKRC(upload(&p->d_sidx, h.sidx, bytes));
+    CHECKRC(upload(&p->d_srowptr, h.srowptr, bytes));
+    HIPOK(hipMalloc((void **)&p->d_queue, 2 * sizeof(int32_t)));
+    HIPOK(hipMemset(p->d_queue, 0, 2 * sizeof(int32_t)));
+    HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->own_stream = true;
+    p->grid = occupancy_grid(p->rtc, p->fn, p->aot, o.wg, std::max(1, p->stats.ntiles), o.grid);
+    p->stats.rtc_used = p->rtc ? 1 : 0;
+    p->stats.grid = p->grid;
+    p->stats.device_bytes = bytes;
+    p->device_bytes = bytes;
+    // the big arrays live on the device now; keep only descriptors on the host
+    p->host.dval = std::vector<float>();
+    p->host.dhi = std::vector<uint16_t>();
+    p->host.dlo = std::vector<uint16_t>();
+    p->host.sval = std::vector<float>();
+    p->host.sidx = std::vector<uint16_t>();
+    p->host.srowptr = std::vector<int32_t>();
+    p->keep_host = false;
+    *out = p.release();
+    return SABLE_OK;
+}
+
+int sable_plan_destroy(SablePlan *p)
+{
+    if (!p) return SABLE_OK;
+    if (p->stream) hipStreamSynchronize(p->stream);
+    void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_dhi, p->d_dlo, p->d_sval,
+                    p->d_sidx, p->d_srowptr, p->d_queue, p->d_xptr, p->d_yptr, p->d_clock,
+                    p->d_xtmp, p->d_ytmp, p->d_flush};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
+    delete p;
+    return SABLE_OK;
+}
+
+int sable_plan_stats(const SablePlan *p, SablePlanStats *out)
+{
+    if (!p || !out) return fail(SABLE_ERR_INVALID, "null argument");
+    *out = p->stats;
+    return SABLE_OK;
+}
+
+int sable_plan_set_stream(SablePlan *p, void *stream)
+{
+    if (!p || p->host_only) return fail(SABLE_ERR_INVALID, "invalid plan");
+    HIPOK(hipStreamSynchronize(p->stream));
+    if (p->own_stream) hipStreamDestroy(p->stream);
+    p->stream = (hipStream_t)stream;
+    p->own_stream = false;
+    return SABLE_OK;
+}
+
+int sable_plan_export(const SablePlan *p, int which, void *dst, int64_t *bytes)
+{
+    if (!p || !bytes) return fail(SABLE_ERR_INVALID, "null argument");
+    const sable::HostPlan &h = p->host;
+    const void *hsrc = nullptr;
+    const void *dsrc = nullptr;
+    int64_t n = 0;
+    switch (which) {
+    case 0: hsrc = h.tiles.data(); n = h.tiles.size() * sizeof(SbTile); break;
+    case 1: hsrc = h.pieces.data(); n = h.pieces.size() * sizeof(SbPiece); break;
+    case 2: hsrc = h.xr.data(); n = h.xr.size() * sizeof(SbXRange); break;
+    case 3: hsrc = h.dval.data(); dsrc = p->d_dval; n = (p->keep_host ? h.dval.size() : p->stats.dense_padded) * 4; break;
+    case 4: hsrc = h.sval.data(); dsrc = p->d_sval; n = p->stats.sparse_nnz * 4; break;
+    case 5: hsrc = h.sidx.data(); dsrc = p->d_sidx; n = p->stats.sparse_nnz * 2; break;
+    case 6: hsrc = h.srowptr.data(); dsrc = p->d_srowptr; n = p->stats.rowptr_entries * 4; break;
+    case 7: hsrc = h.dhi.data(); dsrc = p->d_dhi; n = p->stats.mfma_padded * 2; break;
+    case 8: hsrc = h.dlo.data(); dsrc = p->d_dlo; n = p->stats.mfma_padded * 2; break;
+    default: return fail(SABLE_ERR_INVALID, "unknown export id");
+    }
+    *bytes = n;
+    if (!dst || n == 0) return SABLE_OK;
+    if (p->keep_host || which <= 2) {
+        std::memcpy(dst, hsrc, n);
+    } else {
+        HIPOK(hipMemcpy(dst, dsrc, n, hipMemcpyDeviceToHost));
+    }
+    return SABLE_OK;
+}
+
+static int run_op(SablePlan *p, const float *x, float *y, int flags, int64_t nx, int64_t ny)
+{
+    if (!p) return fail(SABLE_ERR_INVALID, "null plan");
+    if (p->host_only) return fail(SABLE_ERR_INVALID, "host-only plan cannot run");
+    if (!x || !y) return fail(SABLE_ERR_INVALID, "null x/y");
+    const float *dx = x;
+    float *dy = y;
+    if (flags & SABLE_HOST_PTRS) {
+        if (p->xtmp_n < nx) {
+            if (p->d_xtmp) hipFree(p->d_xtmp);
+            p->d_xtmp = nullptr;
+            HIPOK(hipMalloc((void **)&p->d_xtmp, std::max<int64_t>(nx, 1) * 4));
+            p->xtmp_n = nx;
+        }
+        if (p->ytmp_n < ny) {
+            if (p->d_ytmp) hipFree(p->d_ytmp);
+            p->d_ytmp = nullptr;
+            HIPOK(hipMalloc((void **)&p->d_ytmp, std::max<int64_t>(ny, 1) * 4));
+            p->ytmp_n = ny;
+        }
+        HIPOK(hipMemcpyAsync(p->d_xtmp, x, nx * 4, hipMemcpyHostToDevice, p->stream));
+        dx = p->d_xtmp;
+        dy = p->d_ytmp;
+    }
+    CHECKRC(plan_launch(p, dx, dy));
+    HIPOK(hipGetLastError());
+    if (flags & SABLE_HOST_PTRS) {
+        HIPOK(hipMemcpyAsync(y, dy, ny * 4, hipMemcpyDeviceToHost, p->stream));
+        HIPOK(hipStreamSynchronize(p->stream));
+    } else if (!(flags & SABLE_NO_SYNC)) {
+        HIPOK(hipStreamSynchronize(p->stream));
+    }
+    return SABLE_OK;
+}
+
+int sable_spmv(SablePlan *p, const float *x, float *y, int flags)
+{
+    if (p && p->host.op != SABLE_OP_SPMV) return fail(SABLE_ERR_INVALID, "plan was built for SpMM");
+    return run_op(p, x, y, flags, p ? p->stats.ncols : 0, p ? p->stats.nrows : 0);
+}
+
+int sable_spmm(SablePlan *p, const float *X, float *Y, int flags)
+{
+    if (p && p->host.op != SABLE_OP_SPMM) return fail(SABLE_ERR_INVALID, "plan was built for SpMV");
+    return run_op(p, X, Y, flags, p ? (int64_t)p->stats.ncols * p->stats.n_rhs : 0,
+                  p ? (int64_t)p->stats.nrows * p->stats.n_rhs : 0);
+}
+
+int sable_time(SablePlan *p, const float *x, float *y, int warmup, int reps, int64_t flush_bytes,
+               float *times_ms)
+{
+    if (!p || p->host_only) return fail(SABLE_ERR_INVALID, "invalid plan");
+    if (reps < 0 || warmup < 0 || (reps > 0 && !times_ms)) return fail(SABLE_ERR_INVALID, "bad reps");
+    if (flush_bytes > 0) CHECKRC(ensure_flush(&p->d_flush, &p->flush_n, flush_bytes));
+    for (int i = 0; i < warmup; ++i) {
+        if (flush_bytes > 0) CHECKRC(flush_caches(p->d_flush, p->flush_n, p->stream, (uint32_t)i));
+        CHECKRC(plan_launch(p, x, y));
+    }
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    int rc = SABLE_OK;
+    for (int r = 0; r < reps && rc == SABLE_OK; ++r) {
+        if (flush_bytes > 0) rc = flush_caches(p->d_flush, p->flush_n, p->stream, (uint32_t)(r + 7));
+        if (rc) break;
+        if (hipEventRecord(e0, p->stream) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event record"); break; }
+        rc = plan_launch(p, x, y);
+        if (rc) break;
+        if (hipEventRecord(e1, p->stream) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event record"); break; }
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event sync"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        times_ms[r] = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    HIPOK(hipStreamSynchronize(p->stream));
+    return SABLE_OK;
+}
+
+int sable_plan_enable_tile_clock(SablePlan *p, int enable)
+{
+    if (!p || p->host_only) return fail(SABLE_ERR_INVALID, "invalid plan");
+    if (enable && !p->d_clock && p->stats.ntiles > 0) {
+        HIPOK(hipMalloc((void **)&p->d_clock, 2 * sizeof(uint64_t) * p->stats.ntiles));
+        HIPOK(hipMemset(p->d_clock, 0, 2 * sizeof(uint64_t) * p->stats.ntiles));
+    }
+    p->clock_on = enable != 0;
+    return SABLE_OK;
+}
+
+int sable_tile_times(SablePlan *p, float *ns, int32_t ntiles)
+{
+    if (!p || !p->d_clock || !ns) return fail(SABLE_ERR_INVALID, "tile clock not enabled");
+    if (ntiles != p->stats.ntiles) return fail(SABLE_ERR_INVALID, "ntiles mismatch");
+    std::vector<uint64_t> c(2 * (size_t)ntiles);
+    HIPOK(hipStreamSynchronize(p->stream));
+    HIPOK(hipMemcpy(c.data(), p->d_clock, c.size() * 8, hipMemcpyDeviceToHost));
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g_device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    const double ns_per_tick = 1e6 / (double)rate_khz;
+    // tiles are stored in cost order; report in that same (plan) order
+    for (int32_t t = 0; t < ntiles; ++t) ns[t] = (float)((double)(c[2 * t + 1] - c[2 * t]) * ns_per_tick);
+    return SABLE_OK;
+}
+
+// ---- groups: many plans in one persistent launch, captured in a hipGraph
+
+int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x, float *const *y,
+                       int mode, int grid, SableGroup **out)
+{
+    if (!plans || nplans <= 0 || !x || !y || !out) return fail(SABLE_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    std::unique_ptr<SableGroup> g(new SableGroup());
+    g->op = plans[0]->host.op;
+    g->n_rhs = plans[0]->host.n_rhs;
+    g->mode = (mode < 0 || mode > 2) ? SABLE_MODE_QUEUE : mode;
+    // concatenate descriptors; data arrays are concatenated on the device
+    std::vector<SbTile> tiles;
+    std::vector<SbPiece> pieces;
+    std::vector<SbXRange> xr;
+    std::vector<double> cost;
+    int64_t dval_n = 0, dhi_n = 0, sval_n = 0, srow_n = 0;
+    int maxcols = 1, wg = plans[0]->opts.wg;
+    bool mfma = false;
+    for (int i = 0; i < nplans; ++i) {
+        SablePlan *p = plans[i];
+        if (!p || p->host_only) return fail(SABLE_ERR_INVALID, "group member must be a device plan");
+        if (p->host.op != g->op || p->host.n_rhs != g->n_rhs || p->opts.wg != wg)
+            return fail(SABLE_ERR_INVALID, "group members must share op, n_rhs and wg");
+        const int32_t pbase = (int32_t)pieces.size(), xbase = (int32_t)xr.size();
+        for (auto pc : p->host.pieces) {
+            pc.xr_begin += xbase;
+            pc.xr_end += xbase;
+            if (pc.kind == SB_PIECE_DENSE) pc.data_off += dval_n;
+            else if (pc.kind == SB_PIECE_DENSE_BF16) { pc.data_off += dhi_n; mfma = true; }
+            else { pc.data_off += sval_n; pc.rowptr_off += (int32_t)srow_n; }
+            maxcols = std::max(maxcols, pc.ncols);
+            pieces.push_back(pc);
+        }
+        xr.insert(xr.end(), p->host.xr.begin(), p->host.xr.end());
+        for (size_t t = 0; t < p->host.tiles.size(); ++t) {
+            SbTile tt = p->host.tiles[t];
+            tt.piece_begin += pbase;
+            tt.piece_end += pbase;
+            tt.mat = i;
+            tiles.push_back(tt);
+            cost.push_back(p->host.tile_cost[t]);
+        }
+        dval_n += p->stats.dense_padded;
+        dhi_n += p->stats.mfma_padded;
+        sval_n += p->stats.sparse_nnz;
+        srow_n += p->stats.rowptr_entries;
+    }
+    // global LPT order
+    std::vector<int32_t> order(tiles.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    std::vector<SbTile> sorted(tiles.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
+    g->ntiles = (int32_t)sorted.size();
+    int64_t bytes = 0;
+    CHECKRC(upload(&g->d_tiles, sorted, bytes));
+    CHECKRC(upload(&g->d_pieces, pieces, bytes));
+    CHECKRC(upload(&g->d_xr, xr, bytes));
+    auto cat = [&](auto **dst, int64_t n, auto getter, size_t esz) -> int {
+        *dst = nullptr;
+        if (n == 0) return SABLE_OK;
+        HIPOK(hipMalloc((void **)dst, n * esz));
+        int64_t off = 0;
+        for (int i = 0; i < nplans; ++i) {
+            auto pr = getter(plans[i]);
+            if (pr.second) {
+                HIPOK(hipMemcpy((char *)*dst + off * esz, pr.first, pr.second * esz, hipMemcpyDeviceToDevice));
+                off += pr.second;
+            }
+        }
+        return SABLE_OK;
+    };
+    using P2 = std::pair<const void *, int64_t>;
+    CHECKRC(cat(&g->d_dval, dval_n, [](SablePlan *p) { return P2(p->d_dval, p->stats.dense_padded); }, 4));
+    CHECKRC(cat(&g->d_dhi, dhi_n, [](SablePlan *p) { return P2(p->d_dhi, p->stats.mfma_padded); }, 2));
+    CHECKRC(cat(&g->d_dlo, dhi_n, [](SablePlan *p) { return P2(p->d_dlo, p->stats.mfma_padded); }, 2));
+    CHECKRC(cat(&g->d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_nnz); }, 4));
+    CHECKRC(cat(&g->d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_nnz); }, 2));
+    CHECKRC(cat(&g->d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
+    HIPOK(hipMalloc((void **)&g->d_queue, 2 * sizeof(int32_t)));
+    HIPOK(hipMemset(g->d_queue, 0, 2 * sizeof(int32_t)));
+    HIPOK(hipMalloc((void **)&g->d_xptr, nplans * sizeof(float *)));
+    HIPOK(hipMalloc((void **)&g->d_yptr, nplans * sizeof(float *)));
+    HIPOK(hipMemcpy(g->d_xptr, x, nplans * sizeof(float *), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(g->d_yptr, y, nplans * sizeof(float *), hipMemcpyHostToDevice));
+    // kernel: reuse the specialisation machinery through a throwaway host plan view
+    sable::HostPlan view;
+    view.op = g->op;
+    view.n_rhs = g->n_rhs;
+    view.tiles = sorted;
+    view.pieces = pieces;
+    CHECKRC(select_kernel(view, plans[0]->opts, g->rtc, g->fn, g->aot));
+    g->grid = g->mode == SABLE_MODE_GRID ? g->ntiles
+                                          : occupancy_grid(g->rtc, g->fn, g->aot, wg, std::max(1, g->ntiles), grid);
+    for (int i = 0; i < nplans; ++i) g->plans.push_back(plans[i]);
+    HIPOK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    // capture the single persistent launch
+    HIPOK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc;
+    if (g->op == SABLE_OP_SPMV) {
+        SbSpmvArgs a{};
+        a.tiles = g->d_tiles; a.pieces = g->d_pieces; a.xr = g->d_xr; a.dval = g->d_dval;
+        a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr;
+        a.xvec = g->d_xptr; a.yvec = g->d_yptr; a.queue = g->d_queue;
+        a.ntiles = g->ntiles; a.mode = g->mode;
+        rc = launch_raw(g->rtc, g->fn, g->aot, g->grid, wg, &a, g->stream);
+    } else {
+        SbSpmmArgs a{};
+        a.tiles = g->d_tiles; a.pieces = g->d_pieces; a.xr = g->d_xr; a.dval = g->d_dval;
+        a.dhi = g->d_dhi; a.dlo = g->d_dlo;
+        a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr;
+        a.xmat = g->d_xptr; a.ymat = g->d_yptr; a.queue = g->d_queue;
+        a.ntiles = g->ntiles; a.mode = g->mode; a.n_rhs = g->n_rhs;
+        rc = launch_raw(g->rtc, g->fn, g->aot, g->grid, wg, &a, g->stream);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(g->stream, &graph);
+    if (rc) return rc;
+    if (ce != hipSuccess) return fail(SABLE_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    g->graph = graph;
+    HIPOK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+    return SABLE_OK;
+}
+
+int sable_group_launch(SableGroup *g, int sync)
+{
+    if (!g || !g->exec) return fail(SABLE_ERR_INVALID, "invalid group");
+    HIPOK(hipGraphLaunch(g->exec, g->stream));
+    if (sync) HIPOK(hipStreamSynchronize(g->stream));
+    return SABLE_OK;
+}
+
+int sable_group_time(SableGroup *g, int warmup, int reps, int64_t flush_bytes, float *times_ms)
+{
+    if (!g || !g->exec) return fail(SABLE_ERR_INVALID, "invalid group");
+    if (reps > 0 && !times_ms) return fail(SABLE_ERR_INVALID, "null times");
+    if (flush_bytes > 0) CHECKRC(ensure_flush(&g->d_flush, &g->flush_n, flush_bytes));
+    for (int i = 0; i < warmup; ++i) {
+        if (flush_bytes > 0) CHECKRC(flush_caches(g->d_flush, g->flush_n, g->stream, (uint32_t)i));
+        HIPOK(hipGraphLaunch(g->exec, g->stream));
+    }
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    for (int r = 0; r < reps; ++r) {
+        if (flush_bytes > 0) CHECKRC(flush_caches(g->d_flush, g->flush_n, g->stream, (uint32_t)(r + 3)));
+        HIPOK(hipEventRecord(e0, g->stream));
+        HIPOK(hipGraphLaunch(g->exec, g->stream));
+        HIPOK(hipEventRecord(e1, g->stream));
+        HIPOK(hipEventSynchronize(e1));
+        hipEventElapsedTime(&times_ms[r], e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return SABLE_OK;
+}
+
+int sable_group_destroy(SableGroup *g)
+{
+    if (!g) return SABLE_OK;
+    if (g->stream) hipStreamSynchronize(g->stream);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    void *bufs[] = {g->d_tiles, g->d_pieces, g->d_xr, g->d_dval, g->d_dhi, g->d_dlo, g->d_sval,
+                    g->d_sidx, g->d_srowptr, g->d_queue, g->d_xptr, g->d_yptr, g->d_flush};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return SABLE_OK;
+}
+
+// ---- router MLP
+
+int sable_mlp_forward(const float *params, int F, int H, int K, const float *X, int n, float *out,
+                      float *hidden)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (F <= 0 || H <= 0 || K <= 0 || n < 0 || H > 256 || F > 64 || K > 64)
+        return fail(SABLE_ERR_INVALID, "MLP sizes out of range (F<=64, H<=256, K<=64)");
+    if (n == 0) return SABLE_OK;
+    hipLaunchKernelGGL(sb_mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), 0, nullptr, params, F, H, K,
+                       X, n, out, hidden);
+    HIPOK(hipGetLastError());
+    HIPOK(hipDeviceSynchronize());
+    return SABLE_OK;
+}
+
+int sable_mlp_backward(const float *params, int F, int H, int K, const float *X, int n,
+                       const float *hidden, const float *dout, float *grad)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (F <= 0 || H <= 0 || K <= 0 || n < 0 || H > 256 || F > 64 || K > 64)
+        return fail(SABLE_ERR_INVALID, "MLP sizes out of range (F<=64, H<=256, K<=64)");
+    const int64_t np = (int64_t)H * F + H + (int64_t)K * H + K;
+    // one workgroup per parameter chunk; deterministic fixed-order sums over samples
+    hipLaunchKernelGGL(sb_mlp_backward_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, nullptr, params,
+                       F, H, K, X, n, hidden, dout, grad);
+    HIPOK(hipGetLastError());
+    HIPOK(hipDeviceSynchronize());
+    return SABLE_OK;
+}
+
+int sable_mlp_mse_grad(const float *out, const float *target, const float *mask, int64_t n, float scale,
+                       float *dout)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (n <= 0) return SABLE_OK;
+    hipLaunchKernelGGL(sb_mlp_mse_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, out,
+                       target, mask, n, scale, dout);
+    HIPOK(hipGetLastError());
+    HIPOK(hipDeviceSynchronize());
+    return SABLE_OK;
+}
+
+int sable_adam_step(float *params, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
+                    float beta2, float eps, int step)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (n <= 0 || step <= 0) return fail(SABLE_ERR_INVALID, "bad Adam arguments");
+    hipLaunchKernelGGL(sb_adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, params, grad, m, v,
+                       n, lr, beta1, beta2, eps, step);
+    HIPOK(hipGetLastError());
+    HIPOK(hipDeviceSynchronize());
+    return SABLE_OK;
+}
+
+// ---- device memory helpers
+
+int sable_malloc(void **ptr, int64_t bytes)
+{
+    if (!ptr || bytes < 0) return fail(SABLE_ERR_INVALID, "bad malloc arguments");
+    HIPOK(hipMalloc(ptr, std::max<int64_t>(bytes, 1)));
+    return SABLE_OK;
+}
+
+int sable_free(void *ptr)
+{
+    if (ptr) HIPOK(hipFree(ptr));
+    return SABLE_OK;
+}
+
+int sable_memcpy_h2d(void *dst, const void *src, int64_t bytes)
+{
+    HIPOK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SABLE_OK;
+}
+
+int sable_memcpy_d2h(void *dst, const void *src, int64_t bytes)
+{
+    HIPOK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SABLE_OK;
+}
+
+int sable_synchronize(void)
+{
+    HIPOK(hipDeviceSynchronize());
+    return SABLE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// Device-visible plan descriptors shared by the host plan builder and the
+// kernels.  Plain C layout (no HIP types) so the builder compiles with g++.
+#pragma once
+#ifdef __HIPCC_RTC__
+// hipRTC: fixed-width types without system headers (LP64, as on the host)
+typedef signed char int8_t;
+typedef int int32_t;
+typedef long int64_t;
+typedef unsigned char uint8_t;
+typedef unsigned short uint16_t;
+typedef unsigned int uint32_t;
+typedef unsigned long uint64_t;
+#else
+#include <stdint.h>
+#endif
+
+#define SB_PIECE_DENSE 0        // column-major hs x ncols FP32 sub-panel
+#define SB_PIECE_SPARSE 1       // CSR over hs rows, uint16 local columns
+#define SB_PIECE_DENSE_BF16 2   // MFMA-ordered bf16 hi/lo sub-panel (SpMM)
+
+// One unit of work: a row segment [row0, row0+nrows) of one block row of one
+// matrix.  The tile owns those y rows (y is written once, by this tile).
+struct SbTile {
+    int32_t row0;
+    int32_t nrows;
+    int32_t piece_begin;     // pieces[piece_begin, piece_end)
+    int32_t piece_end;
+    int32_t mat;             // matrix slot (batched plans); 0 otherwise
+    int32_t dense_tpar;      // T': active lanes of the flat dense stream
+    int32_t dense_mcols;     // M : columns advanced per stream step (4T' = M*nrows)
+    int32_t sp_lanes_log2;   // lanes per sparse row = 1 << sp_lanes_log2
+};
+
+// A panel piece of a tile: the tile's rows restricted to a group of blocks
+// whose columns (ncols <= the x staging capacity) are staged in LDS.
+struct SbPiece {
+    int64_t data_off;        // dense: float offset (multiple of 4) into dval
+                             // sparse: entry offset into sval/sidx
+                             // bf16: element offset (multiple of 8) into dhi/dlo
+    int32_t kind;            // SB_PIECE_*
+    int32_t ncols;           // staged columns (x slice length)
+    int32_t xr_begin;        // staging ranges xr[xr_begin, xr_end)
+    int32_t xr_end;
+    int32_t rowptr_off;      // sparse: srowptr[rowptr_off .. +nrows] (relative to data_off)
+    int32_t ksteps;          // bf16: number of 32-wide k steps (ncols padded)
+};
+
+// A contiguous run of x (or X rows) copied into the staging buffer.
+struct SbXRange {
+    int32_t col0;
+    int32_t len;
+};

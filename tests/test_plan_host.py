@@ -1,0 +1,172 @@
+"""The native plan builder on the CPU (host-only plans): the HBM layout it
+produces must reconstruct the matrix exactly, cover every row once, respect
+the kernels' launch assumptions, and reject malformed VBR input."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sable_amd import synth
+from sable_amd.convert import vbr_to_dense, threshold_routes, ROUTE_DENSE, ROUTE_SPARSE
+
+_lib = pytest.importorskip("sable_amd._lib")
+from sable_amd.runtime import Plan  # noqa: E402
+from sable_amd._lib import SableError  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def bf16_rne(x):
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_to_f32(h):
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+def reconstruct(plan):
+    tiles = plan.export("tiles")
+    pieces = plan.export("pieces")
+    xr = plan.export("xranges")
+    dval = plan.export("dval")
+    sval = plan.export("sval")
+    sidx = plan.export("sidx")
+    srow = plan.export("srowptr")
+    dhi = plan.export("dhi")
+    dlo = plan.export("dlo")
+    M = np.zeros((plan.nrows, plan.ncols), dtype=np.float64)
+    Mb = np.zeros_like(M)   # bf16 hi+lo parts
+    cover = np.zeros(plan.nrows, dtype=np.int64)
+    for t in tiles:
+        r0, hs = int(t["row0"]), int(t["nrows"])
+        cover[r0:r0 + hs] += 1
+        for p in pieces[t["piece_begin"]:t["piece_end"]]:
+            cols = np.concatenate([np.arange(c0, c0 + n) for c0, n in xr[p["xr_begin"]:p["xr_end"]]])
+            assert len(cols) == p["ncols"]
+            off = int(p["data_off"])
+            if p["kind"] == 0:
+                assert off % 4 == 0
+                blk = dval[off:off + hs * len(cols)].reshape(len(cols), hs).T
+                M[r0:r0 + hs][:, cols] += blk
+            elif p["kind"] == 1:
+                rp = srow[p["rowptr_off"]:p["rowptr_off"] + hs + 1]
+                for r in range(hs):
+                    e = slice(off + rp[r], off + rp[r + 1])
+                    lc = sidx[e].astype(np.int64)
+                    assert np.all(np.diff(lc) > 0)  # ascending columns in a row
+                    M[r0 + r, cols[lc]] += sval[e]
+            else:
+                ks = int(p["ksteps"])
+                assert ks == (len(cols) + 31) // 32
+                mt = (hs + 15) // 16
+                for r in range(hs):
+                    for c in range(len(cols)):
+                        lane = r % 16 + 16 * ((c % 32) // 8)
+                        o = off + (((r // 16) * ks + c // 32) * 64 + lane) * 8 + c % 8
+                        Mb[r0 + r, cols[c]] += float(bf16_to_f32(dhi[o:o + 1])[0]) + float(bf16_to_f32(dlo[o:o + 1])[0])
+                # padding stays zero
+                n = mt * ks * 64 * 8
+                assert off + n <= len(dhi)
+    return M, Mb, cover
+
+
+def _small_mixed(seed, n=900, parts=12):
+    vbr, x, cls = synth.mixed_vbr(n, parts, 20, 140, 0.3, 0.5, (0.8, 1.0), (0.02, 0.3), seed)
+    return vbr, x
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_host_plan_reconstructs_matrix(seed):
+    vbr, x = _small_mixed(seed)
+    routes = threshold_routes(vbr.val, vbr.indx, 15)
+    plan = Plan.from_vbr(vbr, routes, host_only=True, tile_bytes=4096)
+    M, _, cover = reconstruct(plan)
+    D = vbr_to_dense(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    assert np.array_equal(M, D.astype(np.float32).astype(np.float64))
+    assert np.all(cover == 1)
+    st = plan.stats
+    assert st["nnz"] == int(np.count_nonzero(vbr.val))
+    assert st["sparse_nnz"] == int(np.count_nonzero(vbr.val[np.repeat(routes == ROUTE_SPARSE, np.diff(vbr.indx))]))
+    assert st["nblocks_dense"] + st["nblocks_sparse"] == len(vbr.bindx)
+
+
+def test_host_plan_launch_invariants():
+    vbr, x = _small_mixed(5, n=1500, parts=30)
+    plan = Plan.from_vbr(vbr, threshold_routes(vbr.val, vbr.indx, 15), host_only=True, tile_bytes=2048)
+    for t in plan.export("tiles"):
+        hs, tp, M = int(t["nrows"]), int(t["dense_tpar"]), int(t["dense_mcols"])
+        assert 1 <= hs <= 256
+        assert 4 * tp == M * hs and 0 < tp <= 256
+        assert 0 <= t["sp_lanes_log2"] <= 6
+    for p in plan.export("pieces"):
+        assert p["ncols"] <= 4096
+
+
+def test_host_plan_splits_wide_and_tall_blocks():
+    """Blocks wider than the staging capacity are split by columns, block
+    rows taller than the workgroup by rows; the layout still reconstructs."""
+    rng = np.random.default_rng(3)
+    h, w = 300, 70
+    val = rng.uniform(-1, 1, h * w).astype(np.float32)
+    val[rng.random(h * w) < 0.5] = 0
+    rp, cp = [0, h], [0, w]
+    args = (val, [0, h * w], [0], rp, cp, [0], [1])
+    for route in (ROUTE_DENSE, ROUTE_SPARSE):
+        plan = Plan(*args, routes=[route], host_only=True, xcap=16)
+        M, _, cover = reconstruct(plan)
+        assert np.array_equal(M, val.reshape(w, h).T.astype(np.float64))
+        assert np.all(cover == 1)
+        assert all(p["ncols"] <= 16 for p in plan.export("pieces"))
+
+
+def test_host_plan_empty_rows_and_unit_blocks():
+    vbr, x = synth.vbr_matrix_gen_seeded(60, 50, 20, 20, 25, seed=7)   # sizes may be 1
+    assert np.any(np.asarray(vbr.bpntrb) == -1)
+    plan = Plan.from_vbr(vbr, None, host_only=True)
+    M, _, cover = reconstruct(plan)
+    D = vbr_to_dense(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    assert np.array_equal(M, D) and np.all(cover == 1)
+
+
+def test_host_plan_mfma_layout():
+    vbr, x = _small_mixed(11, n=400, parts=6)
+    routes = np.where(threshold_routes(vbr.val, vbr.indx, 50) == ROUTE_DENSE, 2, 1).astype(np.uint8)
+    plan = Plan.from_vbr(vbr, routes, op="spmm", n_rhs=16, host_only=True)
+    M, Mb, cover = reconstruct(plan)
+    D = vbr_to_dense(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    assert np.all(cover == 1)
+    tot = M + Mb
+    assert np.allclose(tot, D, rtol=2 ** -15, atol=0)
+    assert all(t["nrows"] <= 64 for t in plan.export("tiles"))
+    # hi parts are the round-to-nearest-even bf16 of the values
+    dhi = plan.export("dhi")
+    assert dhi.dtype == np.uint16 and len(dhi) == plan.stats["mfma_padded"]
+
+
+def test_bad_vbr_rejected():
+    good = dict(val=np.ones(4, np.float32), indx=[0, 4], bindx=[0], rpntr=[0, 2], cpntr=[0, 2],
+                bpntrb=[0], bpntre=[1])
+    Plan(**good, host_only=True)
+    bad = [dict(good, indx=[0, 3]), dict(good, rpntr=[0, 0]), dict(good, bindx=[1]),
+           dict(good, bpntrb=[-1]), dict(good, bpntre=[2])]
+    for b in bad:
+        with pytest.raises((SableError, ValueError)):
+            Plan(**b, host_only=True)
+    # unsorted bindx within a row
+    with pytest.raises(SableError):
+        Plan(np.ones(8, np.float32), [0, 4, 8], [1, 0], [0, 2], [0, 2, 4], [0], [2], host_only=True)
+    # SpMM width must be a multiple of 4
+    with pytest.raises(SableError):
+        Plan(**good, op="spmm", n_rhs=6, host_only=True)
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "sable.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*(sable_\w+)\(", hdr, re.M))
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(_lib.lib, name), name
+    assert declared <= set(_lib.EXPORTED)
