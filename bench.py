@@ -1,0 +1,182 @@
+"""Benchmark of the VBR SpMV/SpMM hot path on one MI355X.
+
+Default workload (the headline): FP32 SpMV on the 100,000 x 100,000 VBR
+matrix with 1,000 x 1,000 partitions sized 20-200, 1% of blocks stored, half
+of them with density 0.8-1.0 and half 0.01-0.2, values N(0,1), x U[-1,1]
+(seed 2).  A "step" is one SpMV launch over the whole matrix with x and the
+matrix resident in HBM.  By default every step starts cold: a 1 GiB buffer is
+overwritten between launches (outside the timed events), so the 256 MB
+last-level cache holds none of the matrix.
+
+Prints ONE JSON line: value = GFLOP/s (2 * nnz / time), the roofline of the
+kernel (algorithmic bytes of the executed format per launch / measured
+launch time vs 8 TB/s HBM), and the reference CPU path (the restated
+generated-C program, FP64 like the reference) timed on this host's cores on
+a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3] [--warm]
+                    [--routing plain|learned|both] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+FLUSH_BYTES = 1 << 30
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_spmv(vbr, routes, x, y64, ay, budget_s=20.0):
+    """Times the restated reference program (FP64, CSR part + dense block
+    calls, src/codegen.py:293-452) on this host: single thread and all
+    allowed cores.  Returns the baseline dict."""
+    from oracle import c_oracle
+    from sable_amd.convert import compress_with_routes
+    lib_path = None
+    kind_flags = "portable -O3 -mavx2 -mfma -ffast-math"
+    try:
+        d = tempfile.mkdtemp(prefix="sable_oracle_")
+        lib_path = c_oracle.build(native_dir=d)
+        kind_flags = "reference CFLAGS (src/consts.py:7, -march=native)"
+    except Exception as e:  # gcc missing or flags unsupported: portable build
+        log(f"native oracle build failed ({e}); using the portable build")
+        lib_path = None
+    val2, indx2, ub, ip, ind, cv = compress_with_routes(vbr.val, vbr.rpntr, vbr.cpntr, vbr.indx, vbr.bindx,
+                                                        vbr.bpntrb, vbr.bpntre, routes)
+    prog = c_oracle.CProgram(val2, indx2, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre, ub, ip,
+                             ind, cv, dtype=np.float64, lib_path=lib_path)
+    x64 = np.asarray(x, dtype=np.float64)
+    cores = len(os.sched_getaffinity(0))
+    out = {}
+    for label, nthreads in (("1thread", 1), ("allcores", cores)):
+        prog.set_threads(nthreads) if nthreads > 1 else None
+        t0 = time.time()
+        ns, _ = prog.time(x64, bench=1)
+        one = max(float(ns[0]) * 1e-9, 1e-6)
+        bench = int(max(2, min(100, (budget_s / 2) / one)))
+        ns, y = prog.time(x64, bench=bench)
+        med = statistics.median(ns.tolist()) * 1e-9
+        out[label] = {"median_s": med, "bench": bench, "threads": nthreads, "wall_s": time.time() - t0, "y": y}
+    from oracle.c_oracle import scaled_error
+    err = scaled_error(out["allcores"]["y"], y64, ay)["componentwise"]
+    return out, kind_flags, err, cores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--warm", action="store_true", help="no cache flush between launches")
+    ap.add_argument("--routing", default="plain", choices=["plain", "learned", "both"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", type=int, default=2)
+    ap.add_argument("--tile-bytes", type=int, default=0)
+    ap.add_argument("--threshold", type=float, default=15)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from sable_amd import runtime, synth
+    from sable_amd.convert import threshold_routes
+    from oracle import c_oracle
+
+    runtime.init(0)
+    info = runtime.device_info()
+    log(f"device: {info}")
+    t0 = time.time()
+    if args.workload == "c2":
+        vbr, x, _ = synth.workload_c2()
+        desc = ("SpMV FP32, 100000x100000 VBR, 1000x1000 partitions sized 20-200, 1% blocks, "
+                "half density 0.8-1.0 / half 0.01-0.2, N(0,1), seed 2")
+    elif args.workload == "c1":
+        vbr, x = synth.workload_c1()
+        desc = "SpMV FP32, 1000x1000 VBR, 20x20 random partitions, 40 dense blocks, seed 1"
+    else:
+        raise SystemExit("c3 is an SpMM workload: use bench_spmm.py")
+    log(f"workload generated in {time.time() - t0:.1f}s: {len(vbr.val)} stored values, {len(vbr.bindx)} blocks")
+    routes = threshold_routes(vbr.val, vbr.indx, args.threshold)
+    t1 = time.time()
+    plan = runtime.Plan.from_vbr(vbr, routes, mode=args.mode, tile_bytes=args.tile_bytes)
+    inspector_s = time.time() - t1
+    st = plan.stats
+    log(f"plan: {st['ntiles']} tiles, rtc={st['rtc_used']}, grid={st['grid']}, bytes_alg={st['bytes_alg']}, "
+        f"built in {inspector_s:.2f}s")
+    xd = runtime.DeviceArray.from_numpy(x)
+    yd = runtime.DeviceArray(plan.nrows)
+    flush = 0 if args.warm else FLUSH_BYTES
+    times = plan.time(xd, yd, warmup=args.warmup, reps=args.steps, flush_bytes=flush)
+    y = yd.numpy()
+    y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                             vbr.bpntrb, vbr.bpntre, x)
+    errs = c_oracle.scaled_error(y, y64, ay)
+    ms = float(np.mean(times))
+    med = float(np.median(times))
+    gflops = st["flops_alg"] / (ms * 1e-3) / 1e9
+    achieved = st["bytes_alg"] / (ms * 1e-3) / 1e9
+    # warm numbers for reference (the MALL holds part of the matrix)
+    warm = plan.time(xd, yd, warmup=3, reps=max(5, args.steps // 2), flush_bytes=0)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_spmv_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        res, kind_flags, cpu_err, cores = cpu_baseline_spmv(vbr, routes, x, y64, ay)
+        mc = res["allcores"]["median_s"]
+        m1 = res["1thread"]["median_s"]
+        cpu = {"value": st["flops_alg"] / mc / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
+               "sample": (f"whole matrix, FP64 restated generated program (CSR part + dense block calls), "
+                          f"{res['allcores']['bench']} timed passes on {cores} threads (LPT block-row chunks), "
+                          f"median; built with {kind_flags}"),
+               "single_thread_gflops": st["flops_alg"] / m1 / 1e9,
+               "single_thread_passes": res["1thread"]["bench"],
+               "error_vs_fp64_oracle": cpu_err,
+               "gpu_speedup_vs_allcores": mc / (ms * 1e-3)}
+
+    line = {
+        "metric": "VBR SpMV/SpMM GFLOP/s + % HBM/MFMA peak, learned-routed vs plain, 1 GPU",
+        "value": gflops, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "routing": f"plain fixed threshold T={args.threshold:g}",
+                   "cache": "warm" if args.warm else "cold (1 GiB flush between launches)",
+                   "nrows": plan.nrows, "ncols": plan.ncols, "nnz": st["nnz"],
+                   "stored_values": int(len(vbr.val)), "blocks": int(len(vbr.bindx))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "median_ms": med, "p10_ms": float(np.percentile(times, 10)), "p90_ms": float(np.percentile(times, 90)),
+        "warm_ms": float(np.mean(warm)),
+        "warm_gbs": st["bytes_alg"] / (float(np.mean(warm)) * 1e-3) / 1e9,
+        "error_vs_fp64": errs, "bytes_alg": st["bytes_alg"], "inspector_s": inspector_s,
+        "plan": {k: st[k] for k in ("ntiles", "npieces", "dense_values", "sparse_nnz", "rowptr_entries",
+                                    "nblocks_dense", "nblocks_sparse", "rtc_used", "grid", "mode")},
+        "device": info,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -155,8 +155,10 @@ int rtc_compile(const std::string &defines, const std::string &stub, std::vector
 int rtc_code_object(const std::string &defines, const std::string &stub, std::vector<char> &code,
                     std::string &path_out)
 {
-    const std::string key = g_arch + "\n" + defines + stub + k_src_sable_kernels_h + k_src_sable_mfma_h +
-                            k_src_sable_device_h;
+    int vmaj = 0, vmin = 0;
+    hiprtcVersion(&vmaj, &vmin);
+    const std::string key = g_arch + "\n" + std::to_string(vmaj) + "." + std::to_string(vmin) + "\n" + defines +
+                            stub + k_src_sable_kernels_h + k_src_sable_mfma_h + k_src_sable_device_h;
     char name[64];
     snprintf(name, sizeof name, "sb_%016llx.co", (unsigned long long)fnv1a(key));
     const std::string dir = cache_dir();

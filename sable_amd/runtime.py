@@ -4,7 +4,11 @@ A ``Plan`` is the in-process replacement of one generated-and-compiled
 SABLE program (src/codegen.py:293-452): it holds the matrix laid out in HBM
 for one operation (SpMV, or SpMM with a given RHS width) and the kernel
 specialised for it.  x/y may be numpy arrays (copied through host memory) or
-device tensors (``torch`` on ROCm, used only as an allocator).
+``DeviceArray``s (device memory owned by the library's allocator).
+
+The library runs on the system ROCm runtime.  A process that also
+initialises torch's bundled ROCm runtime would hold two HIP/HSA runtimes, so
+the GPU path here never goes through torch.
 """
 from __future__ import annotations
 
@@ -58,8 +62,59 @@ def _vbr_struct(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre):
     return s, keep
 
 
+class DeviceArray:
+    """A device buffer from the library's allocator (sable_malloc)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        init()
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * self.dtype.itemsize
+        p = _lib.P()
+        check(lib.sable_malloc(C.byref(p), self.nbytes))
+        self._p = p
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.copy_from(a)
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float32):
+        return cls.from_numpy(np.zeros(shape, dtype=dtype))
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.size:
+            raise ValueError("size mismatch")
+        check(lib.sable_memcpy_h2d(self._p, a.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib.sable_memcpy_d2h(out.ctypes.data, self._p, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "_p", None) is not None and self._p.value:
+            lib.sable_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _ptr(t, numel=None):
-    """Device pointer of a torch tensor, or host pointer of a numpy array;
+    """Device pointer of a DeviceArray, or host pointer of a numpy array;
     checks dtype float32, contiguity and (when given) the element count so a
     wrong buffer is rejected on the host before any kernel sees it."""
     if isinstance(t, np.ndarray):
@@ -68,12 +123,13 @@ def _ptr(t, numel=None):
         if numel is not None and t.size != numel:
             raise ValueError(f"array has {t.size} elements, expected {numel}")
         return t.ctypes.data, True
-    import torch
-    if t.dtype != torch.float32 or not t.is_contiguous() or t.device.type != "cuda":
-        raise ValueError("device tensors must be contiguous float32 on the GPU")
-    if numel is not None and t.numel() != numel:
-        raise ValueError(f"tensor has {t.numel()} elements, expected {numel}")
-    return t.data_ptr(), False
+    if not isinstance(t, DeviceArray):
+        raise TypeError("x/y must be numpy arrays or DeviceArrays")
+    if t.dtype != np.float32:
+        raise ValueError("device arrays must be float32")
+    if numel is not None and t.size != numel:
+        raise ValueError(f"device array has {t.size} elements, expected {numel}")
+    return t.ptr, False
 
 
 class Plan:
@@ -161,11 +217,7 @@ class Plan:
     def _run(self, fn, x, y, shape_y):
         xp, xhost = _ptr(x, self.ncols * self.n_rhs)
         if y is None:
-            if xhost:
-                y = np.empty(shape_y, dtype=np.float32)
-            else:
-                import torch
-                y = torch.empty(shape_y, dtype=torch.float32, device=x.device)
+            y = np.empty(shape_y, dtype=np.float32) if xhost else DeviceArray(shape_y)
         yp, yhost = _ptr(y, int(np.prod(shape_y)))
         if xhost != yhost:
             raise ValueError("x and y must both be host arrays or both device tensors")
