@@ -145,9 +145,9 @@ int sable_spmm(SablePlan *plan, const float *X, float *Y, int flags);
 int sable_time(SablePlan *plan, const float *x, float *y, int warmup, int reps,
                int64_t flush_bytes, float *times_ms);
 
-/* Per-tile wall-clock durations of the last launch made with clocks enabled
- * (router training labels).  enable=1 turns recording on for later launches;
- * sable_tile_times copies ntiles durations in nanoseconds. */
+/* Per-tile wall-clock durations (no reference counterpart).  Disabled in
+ * this build: both return SABLE_ERR_UNSUPPORTED (router labels come from
+ * event-timed launches instead). */
 int sable_plan_enable_tile_clock(SablePlan *plan, int enable);
 int sable_tile_times(SablePlan *plan, float *ns, int32_t ntiles);
 

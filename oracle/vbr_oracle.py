@@ -347,3 +347,40 @@ def cut_indices2(dense, structural, cut_threshold):
         return cuts
 
     return scan(nc, col, col_nnz), scan(nr, row, row_nnz)
+
+
+def cut_indices2_dense_lines(A, cut_threshold):
+    """src/autopartition.py:34-68 with similarity2 (:73-74), done the way the
+    reference does it: every line is densified (``toarray().ravel()``) and
+    the three dot products are numpy dots.  Usable at real-matrix scale
+    (e.g. the reference's tests/Franz8.mtx); A is a scipy CSC matrix."""
+    import numpy as np
+    A = A.tocsc()
+    R = A.tocsr()
+
+    def sim(a, b):
+        num = max(a.dot(b), a[1:].dot(b[:-1]), a[:-1].dot(b[1:]))
+        den = max(np.count_nonzero(a), np.count_nonzero(b))
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return np.float64(num) / den
+
+    def scan(n, line, nnz):
+        cuts = [0]
+        i = 0
+        while i < n - 1:
+            if nnz(i) == 0:
+                while i < n - 1 and nnz(i + 1) == 0:
+                    i += 1
+                cuts.append(i + 1)
+            elif sim(line(i), line(i + 1)) < cut_threshold:
+                cuts.append(i + 1)
+            i += 1
+        if cuts[-1] != n:
+            cuts.append(n)
+        return cuts
+
+    cols = scan(A.shape[1], lambda i: A[:, i].toarray().ravel().astype(np.float64),
+                lambda i: A.indptr[i + 1] - A.indptr[i])
+    rows = scan(A.shape[0], lambda i: R[i, :].toarray().ravel().astype(np.float64),
+                lambda i: R.indptr[i + 1] - R.indptr[i])
+    return cols, rows

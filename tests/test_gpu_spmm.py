@@ -1,0 +1,139 @@
+"""GPU parity of the SpMM paths (FP32 vector path and bf16 hi+lo MFMA path)
+against the FP64 oracle and the restated reference SpMM program (marked gpu).
+Bar: componentwise-scaled error <= 1e-5 for both paths."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle
+from sable_amd import synth
+from sable_amd.convert import threshold_routes, ROUTE_DENSE, ROUTE_SPARSE, ROUTE_DENSE_MFMA
+from sable_amd.fileio import read_vbr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from sable_amd import runtime
+    runtime.init()
+    return runtime
+
+
+def _err(vbr, X, Y):
+    Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                             vbr.bpntrb, vbr.bpntre, X)
+    return c_oracle.scaled_error(np.asarray(Y).ravel(), Y64.ravel(), AY.ravel())
+
+
+def _c3_like(n=3000, seed=3):
+    vbr, _, cls = synth.mixed_vbr(n, n // 100, 32, 192, 0.05, 0.7, (0.9, 1.0), (0.02, 0.1), seed)
+    return vbr
+
+
+def test_example_canon_n512(rt, golden):
+    """tests/output_spmm_canon.txt: Y = A 1 with the reference's 512 columns."""
+    val, indx, bindx, rp, cp, bb, be = read_vbr(os.path.join(golden, "example.vbr"))
+    with gzip.open(os.path.join(golden, "output_spmm_canon.txt.gz"), "rt") as f:
+        canon = [float(v) for v in f.read().split()]
+    print("case: example n=512 fp32", flush=True)
+    plan = rt.Plan(val, indx, bindx, rp, cp, bb, be, op="spmm", n_rhs=512)
+    Y = plan.spmm(np.ones((11, 512), np.float32))
+    assert [round(float(v), 6) for v in Y.ravel()] == canon
+
+
+@pytest.mark.parametrize("route", [ROUTE_DENSE, ROUTE_DENSE_MFMA])
+@pytest.mark.parametrize("n_rhs", [16, 128])
+def test_dense_routes_match_oracle(rt, route, n_rhs):
+    vbr = _c3_like()
+    X = np.random.default_rng(1).uniform(-1, 1, (vbr.cpntr[-1], n_rhs)).astype(np.float32)
+    routes = np.where(threshold_routes(vbr.val, vbr.indx, 15) == ROUTE_DENSE, route, ROUTE_SPARSE)
+    print(f"case: c3-like route={route} n={n_rhs}", flush=True)
+    plan = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs)
+    Y = plan.spmm(X)
+    e = _err(vbr, X, Y)
+    print(f"  componentwise {e['componentwise']:.3e}", flush=True)
+    assert e["componentwise"] <= TOL
+
+
+@pytest.mark.parametrize("n_rhs", [4, 8, 12, 64, 256])
+def test_fp32_widths(rt, n_rhs):
+    vbr = _c3_like(2000, seed=5)
+    X = np.random.default_rng(2).uniform(-1, 1, (vbr.cpntr[-1], n_rhs)).astype(np.float32)
+    routes = threshold_routes(vbr.val, vbr.indx, 15)
+    print(f"case: fp32 n={n_rhs}", flush=True)
+    Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs).spmm(X)
+    assert _err(vbr, X, Y)["componentwise"] <= TOL
+
+
+@pytest.mark.parametrize("n_rhs", [32, 64, 256])
+def test_mfma_widths(rt, n_rhs):
+    vbr = _c3_like(2000, seed=6)
+    X = np.random.default_rng(3).uniform(-1, 1, (vbr.cpntr[-1], n_rhs)).astype(np.float32)
+    routes = np.where(threshold_routes(vbr.val, vbr.indx, 15) == ROUTE_DENSE, ROUTE_DENSE_MFMA, ROUTE_SPARSE)
+    print(f"case: mfma n={n_rhs}", flush=True)
+    Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs).spmm(X)
+    assert _err(vbr, X, Y)["componentwise"] <= TOL
+
+
+def test_three_routes_in_one_block_row(rt):
+    """MFMA, FP32-dense and sparse blocks share block rows: the MFMA and
+    vector partial results meet in the LDS Y tile."""
+    vbr = _c3_like(2500, seed=9)
+    rng = np.random.default_rng(4)
+    routes = rng.choice([ROUTE_DENSE, ROUTE_SPARSE, ROUTE_DENSE_MFMA], len(vbr.bindx)).astype(np.uint8)
+    X = rng.uniform(-1, 1, (vbr.cpntr[-1], 64)).astype(np.float32)
+    print("case: three routes", flush=True)
+    Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=64).spmm(X)
+    assert _err(vbr, X, Y)["componentwise"] <= TOL
+
+
+def test_mfma_short_and_unit_blocks(rt):
+    """Block rows shorter than an MFMA tile (heights 1..15) and width-1
+    blocks: padding rows/columns of the MFMA layout stay out of Y."""
+    vbr, _ = synth.vbr_matrix_gen_seeded(200, 180, 40, 30, 150, seed=12)
+    X = np.random.default_rng(5).uniform(-1, 1, (180, 32)).astype(np.float32)
+    routes = np.full(len(vbr.bindx), ROUTE_DENSE_MFMA, np.uint8)
+    print("case: mfma unit blocks", flush=True)
+    Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=32).spmm(X)
+    assert _err(vbr, X, Y)["componentwise"] <= TOL
+
+
+def test_matches_reference_spmm_program(rt):
+    """Against the restated reference SpMM program (src/codegen.py:827-937,
+    N as a parameter, density-routed inline)."""
+    vbr = _c3_like(1500, seed=7)
+    X = np.random.default_rng(6).uniform(-1, 1, (vbr.cpntr[-1], 16)).astype(np.float32)
+    route = threshold_routes(vbr.val, vbr.indx, 15)
+    Yc = c_oracle.spmm(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre, route, X, 16)
+    _, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                           vbr.bpntrb, vbr.bpntre, X)
+    for r in (ROUTE_DENSE, ROUTE_DENSE_MFMA):
+        routes = np.where(route == ROUTE_DENSE, r, ROUTE_SPARSE)
+        print(f"case: vs reference program route={r}", flush=True)
+        Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=16).spmm(X)
+        assert c_oracle.scaled_error(Y.ravel(), Yc.astype(np.float64).ravel(), AY.ravel())["componentwise"] <= 2 * TOL
+
+
+def test_spmm_device_arrays_and_modes(rt):
+    vbr = _c3_like(2000, seed=8)
+    X = np.random.default_rng(7).uniform(-1, 1, (vbr.cpntr[-1], 128)).astype(np.float32)
+    routes = np.where(threshold_routes(vbr.val, vbr.indx, 15) == ROUTE_DENSE, ROUTE_DENSE_MFMA, ROUTE_SPARSE)
+    ref = None
+    for mode in (0, 1, 2):
+        print(f"case: spmm mode={mode}", flush=True)
+        plan = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=128, mode=mode)
+        Xd = rt.DeviceArray.from_numpy(X)
+        Yd = rt.DeviceArray((plan.nrows, 128))
+        plan.spmm(Xd, Yd)
+        Y = Yd.numpy()
+        if ref is None:
+            ref = Y
+        assert np.array_equal(Y, ref)
+        t = plan.time(Xd, Yd, warmup=1, reps=3)
+        assert np.all(t > 0)
+    assert _err(vbr, X, ref)["componentwise"] <= TOL
