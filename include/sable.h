@@ -38,7 +38,7 @@ extern "C" {
  * utils/convert_real_to_vbr.py:52-68, plus the MFMA route for SpMM) */
 #define SABLE_ROUTE_DENSE 0       /* FP32 dense block (GEMV for SpMV, FMA for SpMM) */
 #define SABLE_ROUTE_SPARSE 1      /* nonzeros only, in-block CSR */
-#define SABLE_ROUTE_DENSE_MFMA 2  /* SpMM: bf16 hi+lo split on MFMA (FP32 accumulate) */
+#define SABLE_ROUTE_DENSE_MFMA 2  /* SpMM: bf16 split on MFMA (FP32 accumulate) */
 
 #define SABLE_OP_SPMV 0
 #define SABLE_OP_SPMM 1
@@ -82,6 +82,8 @@ typedef struct {
     int32_t unroll;           /* SpMV dense stream unroll (1,2,4,8); 0 = 4 */
     int32_t use_rtc;          /* 1 = hipRTC per-plan specialisation (default) */
     int32_t host_only;        /* 1 = build the host plan only (no device) */
+    int32_t mfma_split;       /* MFMA route: 3 = exact 3-term bf16 split, 6 passes
+                                 (FP32-level, default); 2 = hi+lo, 3 passes */
 } SablePlanOpts;
 
 typedef struct {
@@ -93,7 +95,7 @@ typedef struct {
     int64_t dense_values;     /* sum over dense-routed blocks of r*c */
     int64_t dense_padded;     /* dense values as laid out (padding included) */
     int64_t mfma_values;      /* sum over MFMA-routed blocks of r*c */
-    int64_t mfma_padded;      /* bf16 elements per part, padding included */
+    int64_t mfma_padded;      /* MFMA-ordered FP32 elements, padding included */
     int64_t sparse_nnz;       /* nonzeros stored in sparse pieces */
     int64_t rowptr_entries;   /* sparse row pointer entries */
     int64_t descriptor_bytes; /* tiles + pieces + ranges */
@@ -122,7 +124,8 @@ int sable_plan_destroy(SablePlan *plan);
 int sable_plan_stats(const SablePlan *plan, SablePlanStats *out);
 int sable_plan_set_stream(SablePlan *plan, void *hip_stream);
 /* Host copies of the plan's arrays (for inspection / tests).  which: 0 tiles,
- * 1 pieces, 2 xranges, 3 dval, 4 sval, 5 sidx, 6 srowptr, 7 dhi, 8 dlo.
+ * 1 pieces, 2 xranges, 3 dval (FP32 dense and MFMA pieces), 4 sval, 5 sidx,
+ * 6 srowptr.
  * Returns the byte size in *bytes; copies when dst != NULL. */
 int sable_plan_export(const SablePlan *plan, int which, void *dst, int64_t *bytes);
 
@@ -175,7 +178,7 @@ int sable_adam_step(float *params, const float *grad, float *m, float *v, int64_
 
 /* Build tooling (no reference counterpart): compile one kernel
  * specialisation into the code-object cache without a device.
- * kind: 0 SpMV, 1 SpMM FP32, 2 SpMM MFMA. */
+ * kind: 0 SpMV, 1 SpMM FP32, 2 SpMM MFMA (split 3), 3 SpMM MFMA (split 2). */
 int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int rtmax,
                          const char *arch);
 

@@ -49,7 +49,7 @@ class SablePlanOpts(C.Structure):
     _fields_ = [("op", I32), ("n_rhs", I32), ("wg", I32), ("xcap", I32), ("tile_bytes", I64),
                 ("max_seg_rows", I32), ("min_seg_rows", I32), ("sp_lanes_log2", I32),
                 ("mode", I32), ("grid", I32), ("unroll", I32), ("use_rtc", I32),
-                ("host_only", I32)]
+                ("host_only", I32), ("mfma_split", I32)]
 
 
 class SablePlanStats(C.Structure):

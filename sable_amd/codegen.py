@@ -40,7 +40,7 @@ FILEPATH = pathlib.Path(__file__).resolve().parent
 BASE_PATH = os.path.join(FILEPATH, "..")
 DENSE_DIR = os.path.join(BASE_PATH, "Generated_dense_tensors")
 
-KIND_SPMV, KIND_SPMM_F32, KIND_SPMM_MFMA = 0, 1, 2
+KIND_SPMV, KIND_SPMM_F32, KIND_SPMM_MFMA, KIND_SPMM_MFMA2 = 0, 1, 2, 3
 
 
 def _pow2_at_least(v, lo=16):
@@ -50,7 +50,7 @@ def _pow2_at_least(v, lo=16):
     return p
 
 
-def kernel_spec(plan_host, op, n_rhs, wg=256, unroll=4):
+def kernel_spec(plan_host, op, n_rhs, wg=256, unroll=4, mfma_split=3):
     """The hipRTC specialisation the runtime selects for a plan: (kind, wg,
     unroll, xcap, nrhs, rtmax), mirroring select_kernel in runtime.hip."""
     pieces = plan_host.export("pieces")
@@ -63,18 +63,21 @@ def kernel_spec(plan_host, op, n_rhs, wg=256, unroll=4):
     rows = int(tiles["nrows"].max()) if len(tiles) else 1
     ngroups = max(1, wg // (n_rhs // 4))
     rtmax = max(1, -(-rows // ngroups))
-    return (KIND_SPMM_MFMA if mfma else KIND_SPMM_F32, wg, unroll, xcap, n_rhs, rtmax)
+    kind = (KIND_SPMM_MFMA2 if mfma_split == 2 else KIND_SPMM_MFMA) if mfma else KIND_SPMM_F32
+    return (kind, wg, unroll, xcap, n_rhs, rtmax)
 
 
 def _kernel_source(spec):
     kind, wg, unroll, xcap, nrhs, rtmax = spec
+    split = 2 if kind == KIND_SPMM_MFMA2 else 3
     body = {KIND_SPMV: "sb_spmv_body<SB_WG, SB_UNROLL, SB_XCAP>(a);",
             KIND_SPMM_F32: "sb_spmm_f32_body<SB_WG, SB_RTMAX, SB_NRHS, SB_XCAP>(a);",
-            KIND_SPMM_MFMA: "sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP>(a);"}[kind]
+            KIND_SPMM_MFMA: "sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP, SB_SPLIT>(a);",
+            KIND_SPMM_MFMA2: "sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP, SB_SPLIT>(a);"}[kind]
     args = "SbSpmvArgs" if kind == KIND_SPMV else "SbSpmmArgs"
     return (f"// kernel specialisation for this matrix (compiled by hipRTC for gfx950)\n"
             f"#define SB_WG {wg}\n#define SB_UNROLL {unroll}\n#define SB_XCAP {xcap}\n"
-            f"#define SB_NRHS {nrhs}\n#define SB_RTMAX {rtmax}\n"
+            f"#define SB_NRHS {nrhs}\n#define SB_RTMAX {rtmax}\n#define SB_SPLIT {split}\n"
             f"#include \"sable_kernels.h\"\n#include \"sable_mfma.h\"\n"
             f"extern \"C\" __global__ void __launch_bounds__(SB_WG) sable_program({args} a)\n"
             f"{{ {body} }}\n")
@@ -86,13 +89,14 @@ def _compile(spec):
 
 
 def _emit(dir_name, filename, arrays, meta, op, n_rhs, routes):
+    split = meta.get("mfma_split", 3)
     from .runtime import Plan
     os.makedirs(dir_name, exist_ok=True)
     np.savez(os.path.join(dir_name, filename + ".sable.npz"), routes=routes, meta=np.array(json.dumps(meta)),
              **arrays)
     host = Plan(arrays["val"], arrays["indx"], arrays["bindx"], arrays["rpntr"], arrays["cpntr"],
-                arrays["bpntrb"], arrays["bpntre"], routes, op=op, n_rhs=n_rhs, host_only=True)
-    spec = kernel_spec(host, op, n_rhs)
+                arrays["bpntrb"], arrays["bpntre"], routes, op=op, n_rhs=n_rhs, host_only=True, mfma_split=split)
+    spec = kernel_spec(host, op, n_rhs, mfma_split=split)
     with open(os.path.join(dir_name, filename + ".hip"), "w") as f:
         f.write(_kernel_source(spec))
     _compile(spec)
@@ -144,18 +148,21 @@ def gen_single_threaded_spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, den
     """src/codegen.py:827-937 with the RHS width as a parameter.  Blocks are
     routed inline like the reference (:906-922): with density > 0 a block is
     dense iff its nonzero percentage > density, otherwise every block is
-    dense.  precision "bf16x3" sends dense blocks to the MFMA route."""
+    dense.  precision: "fp32" (vector FP32 path), "bf16x6" (dense blocks on
+    MFMA with the exact 3-term bf16 split, 6 passes) or "bf16x3" (hi+lo
+    split, 3 passes)."""
     t1 = time.time_ns() // 1_000
     val = np.asarray(val)
     if density > 0:
         routes = convert.threshold_routes(val, indx, density)
     else:
         routes = np.zeros(len(bindx), dtype=np.uint8)
-    if precision == "bf16x3":
+    if precision in ("bf16x3", "bf16x6"):
         routes = np.where(routes == convert.ROUTE_DENSE, convert.ROUTE_DENSE_MFMA, routes).astype(np.uint8)
     arrays = dict(val=val, indx=np.asarray(indx), bindx=np.asarray(bindx), rpntr=np.asarray(rpntr),
                   cpntr=np.asarray(cpntr), bpntrb=np.asarray(bpntrb), bpntre=np.asarray(bpntre))
     meta = {"op": "spmm", "n_rhs": int(n_rhs), "bench": 1, "filename": filename,
+            "mfma_split": 2 if precision == "bf16x3" else 3,
             "matrix": os.path.join(DENSE_DIR, f"generated_matrix_{int(cpntr[-1])}x{int(n_rhs)}.matrix")}
     _emit(dir_name, filename, arrays, meta, "spmm", int(n_rhs), routes)
     return time.time_ns() // 1_000 - t1
@@ -186,7 +193,7 @@ def run_program(dir_name, filename, x=None):
     meta, a = load_program(dir_name, filename)
     op, n = meta["op"], meta["n_rhs"]
     plan = Plan(a["val"], a["indx"], a["bindx"], a["rpntr"], a["cpntr"], a["bpntrb"], a["bpntre"], a["routes"],
-                op=op, n_rhs=n)
+                op=op, n_rhs=n, mfma_split=meta.get("mfma_split", 3))
     ncols = int(a["cpntr"][-1])
     if x is None:
         path = meta["vector"] if op == "spmv" else meta["matrix"]

@@ -38,6 +38,7 @@ DEFAULT_DENSITY = 15
 
 ROUTE_DENSE = 0
 ROUTE_SPARSE = 1
+ROUTE_DENSE_MFMA = 2    # SpMM: bf16 hi+lo split on MFMA
 
 
 def block_nonzero_counts(val, indx):
@@ -77,7 +78,7 @@ def compress_with_routes(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, routes)
     nb = len(ix) - 1
     sizes = np.diff(ix)
     nrows, ncols = int(rpntr[-1]), int(cpntr[-1])
-    dense_b = routes == ROUTE_DENSE
+    dense_b = routes != ROUTE_SPARSE      # both dense routes keep the block
     elem_dense = np.repeat(dense_b, sizes)
     val2 = val[elem_dense]
     indx2 = np.concatenate([[0], np.cumsum(sizes[dense_b])]).astype(np.int64)

@@ -9,9 +9,10 @@
 //    reference's routing count, utils/convert_real_to_vbr.py:41-47) as a CSR
 //    over the segment's rows with uint16 column indices into the group's x
 //    slice (entries of a row in ascending global column order);
-//  * MFMA-routed blocks (SpMM) are split into bf16 hi/lo and stored in the
-//    exact register order of v_mfma_f32_16x16x32_bf16's A operand, so every
-//    lane reads its 8 k-values of hi (and of lo) with one 16-byte load.
+//  * MFMA-routed blocks (SpMM) are stored as FP32 in the exact register order
+//    of v_mfma_f32_16x16x32_bf16's A operand (16-row tile m, 32-column k-step
+//    s, lane l: A[16m + l%16][32s + 8(l/16) + j], j = 0..7), so every lane
+//    reads its 8 k-values with two 16-byte loads and splits them in registers.
 //  * column groups never exceed the LDS staging capacity (xcap columns), and
 //    blocks wider than that are split by columns.
 #include "plan.h"
@@ -22,23 +23,6 @@
 #include <numeric>
 
 namespace sable {
-
-uint16_t f32_to_bf16(float f)
-{
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-float bf16_to_f32(uint16_t h)
-{
-    uint32_t u = (uint32_t)h << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
 
 SablePlanOpts normalize_opts(const SablePlanOpts *in)
 {
@@ -55,6 +39,7 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     if (o.sp_lanes_log2 < -1) o.sp_lanes_log2 = -1;
     if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_QUEUE;
     if (o.unroll <= 0) o.unroll = 4;
+    if (o.mfma_split != 2) o.mfma_split = 3;
     return o;
 }
 
@@ -282,19 +267,18 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                 P.pieces.push_back(pc);
                 cost += 4.0 * hs * pc.ncols;
             }
-            // MFMA bf16 hi/lo pieces (SpMM)
+            // MFMA pieces (SpMM): FP32 in A-operand register order
             for (auto &g : mgroups) {
                 SbPiece pc{};
-                pc.kind = SB_PIECE_DENSE_BF16;
+                pc.kind = SB_PIECE_MFMA;
                 pc.ncols = group_width(g);
                 pc.ksteps = (pc.ncols + 31) / 32;
                 const int32_t mt = (hs + 15) / 16;
-                pc.data_off = (int64_t)P.dhi.size();
+                while (P.dval.size() % 8) P.dval.push_back(0.f);
+                pc.data_off = (int64_t)P.dval.size();
                 const int64_t n = (int64_t)mt * pc.ksteps * 64 * 8;
-                P.dhi.resize(P.dhi.size() + n, 0);
-                P.dlo.resize(P.dlo.size() + n, 0);
-                uint16_t *hi = P.dhi.data() + pc.data_off;
-                uint16_t *lo = P.dlo.data() + pc.data_off;
+                P.dval.resize(P.dval.size() + n, 0.f);
+                float *dst = P.dval.data() + pc.data_off;
                 int32_t c = 0;
                 for (auto &cs : g) {
                     const float *bv = v.val + v.indx[cs.k];
@@ -303,11 +287,7 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                         const int32_t ks = c / 32, kk = c % 32;
                         for (int32_t r = 0; r < hs; ++r) {
                             const int32_t lane = (r % 16) + 16 * (kk / 8);
-                            const int64_t off = (((int64_t)(r / 16) * pc.ksteps + ks) * 64 + lane) * 8 + (kk % 8);
-                            const float f = col[r];
-                            const uint16_t bh = f32_to_bf16(f);
-                            hi[off] = bh;
-                            lo[off] = f32_to_bf16(f - bf16_to_f32(bh));
+                            dst[(((int64_t)(r / 16) * pc.ksteps + ks) * 64 + lane) * 8 + (kk % 8)] = col[r];
                         }
                     }
                 }
@@ -396,7 +376,7 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
     S.ntiles = (int32_t)P.tiles.size();
     S.npieces = (int32_t)P.pieces.size();
     S.nxranges = (int32_t)P.xr.size();
-    S.dense_padded = (int64_t)P.dval.size();
+    S.dense_padded = (int64_t)P.dval.size() - S.mfma_padded;
     S.descriptor_bytes = (int64_t)(P.tiles.size() * sizeof(SbTile) + P.pieces.size() * sizeof(SbPiece) +
                                    P.xr.size() * sizeof(SbXRange));
     S.bytes_alg = 4 * S.dense_values + 4 * S.mfma_values + 6 * S.sparse_nnz + 4 * S.rowptr_entries +

@@ -18,9 +18,7 @@ struct HostPlan {
     std::vector<SbTile> tiles;
     std::vector<SbPiece> pieces;
     std::vector<SbXRange> xr;
-    std::vector<float> dval;      // FP32 dense pieces
-    std::vector<uint16_t> dhi;    // bf16 hi (MFMA order)
-    std::vector<uint16_t> dlo;    // bf16 lo (MFMA order)
+    std::vector<float> dval;      // FP32 dense pieces (both layouts)
     std::vector<float> sval;
     std::vector<uint16_t> sidx;
     std::vector<int32_t> srowptr;
@@ -38,8 +36,5 @@ int validate_vbr(const SableVbr &v, std::string &err);
 int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o,
                     HostPlan &out, std::string &err);
 
-// bf16 helpers (round to nearest even; NaN kept quiet)
-uint16_t f32_to_bf16(float f);
-float bf16_to_f32(uint16_t h);
 
 }  // namespace sable

@@ -43,7 +43,12 @@ extern "C" __global__ void __launch_bounds__(256) sb_spmm_f32_aot(SbSpmmArgs a)
 
 extern "C" __global__ void __launch_bounds__(256) sb_spmm_mfma_aot(SbSpmmArgs a)
 {
-    sb_spmm_mfma_body<256, 0, 4096>(a);
+    sb_spmm_mfma_body<256, 0, 4096, 3>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(256) sb_spmm_mfma2_aot(SbSpmmArgs a)
+{
+    sb_spmm_mfma_body<256, 0, 4096, 2>(a);
 }
 
 extern "C" __global__ void sb_fill_u32(uint32_t *p, int64_t n, uint32_t v)
@@ -209,14 +214,16 @@ int rtc_get(const std::string &defines, const std::string &stub, const char *ent
 // specialisations
 
 struct Spec {
-    int wg = 256, unroll = 4, xcap = 4096, nrhs = 0, rtmax = 8;
+    int wg = 256, unroll = 4, xcap = 4096, nrhs = 0, rtmax = 8, split = 3;
 };
 
 std::string spec_defines(const Spec &s)
 {
-    char b[256];
-    snprintf(b, sizeof b, "#define SB_WG %d\n#define SB_UNROLL %d\n#define SB_XCAP %d\n#define SB_NRHS %d\n#define SB_RTMAX %d\n",
-             s.wg, s.unroll, s.xcap, s.nrhs, s.rtmax);
+    char b[320];
+    snprintf(b, sizeof b,
+             "#define SB_WG %d\n#define SB_UNROLL %d\n#define SB_XCAP %d\n#define SB_NRHS %d\n#define SB_RTMAX %d\n"
+             "#define SB_SPLIT %d\n",
+             s.wg, s.unroll, s.xcap, s.nrhs, s.rtmax, s.split);
     return b;
 }
 
@@ -228,7 +235,7 @@ const char *kStubSpmmF32 =
     "{ sb_spmm_f32_body<SB_WG, SB_RTMAX, SB_NRHS, SB_XCAP>(a); }\n";
 const char *kStubSpmmMfma =
     "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_mfma_rtc(SbSpmmArgs a)\n"
-    "{ sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP>(a); }\n";
+    "{ sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP, SB_SPLIT>(a); }\n";
 
 int pow2_at_least(int v)
 {
@@ -255,7 +262,6 @@ struct SablePlan {
     SbPiece *d_pieces = nullptr;
     SbXRange *d_xr = nullptr;
     float *d_dval = nullptr;
-    uint16_t *d_dhi = nullptr, *d_dlo = nullptr;
     float *d_sval = nullptr;
     uint16_t *d_sidx = nullptr;
     int32_t *d_srowptr = nullptr;
@@ -285,7 +291,6 @@ struct SableGroup {
     SbPiece *d_pieces = nullptr;
     SbXRange *d_xr = nullptr;
     float *d_dval = nullptr;
-    uint16_t *d_dhi = nullptr, *d_dlo = nullptr;
     float *d_sval = nullptr;
     uint16_t *d_sidx = nullptr;
     int32_t *d_srowptr = nullptr;
@@ -347,7 +352,8 @@ int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, h
     s.unroll = o.unroll;
     s.xcap = std::min(4096, pow2_at_least(max_piece_cols(h) + 1));
     const bool spmm = h.op == SABLE_OP_SPMM;
-    const bool mfma = spmm && has_kind(h, SB_PIECE_DENSE_BF16);
+    const bool mfma = spmm && has_kind(h, SB_PIECE_MFMA);
+    s.split = o.mfma_split == 2 ? 2 : 3;
     if (spmm) {
         s.nrhs = h.n_rhs;
         int rows = 1;
@@ -355,7 +361,9 @@ int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, h
         const int ngroups = std::max(1, o.wg / (h.n_rhs / 4));
         s.rtmax = std::max(1, (rows + ngroups - 1) / ngroups);
     }
-    if (spmm) aot = mfma ? (const void *)sb_spmm_mfma_aot : (const void *)sb_spmm_f32_aot;
+    if (spmm)
+        aot = mfma ? (s.split == 2 ? (const void *)sb_spmm_mfma2_aot : (const void *)sb_spmm_mfma_aot)
+                   : (const void *)sb_spmm_f32_aot;
     else aot = (const void *)sb_spmv_aot;
     if (!o.use_rtc) {
         if (o.wg != 256 || (!spmm && o.unroll != 4)) return fail(SABLE_ERR_UNSUPPORTED, "AOT kernels exist only for wg=256 / unroll=4");
@@ -418,7 +426,7 @@ int plan_launch(SablePlan *p, const float *x, float *y)
     }
     SbSpmmArgs a{};
     a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
-    a.dval = p->d_dval; a.dhi = p->d_dhi; a.dlo = p->d_dlo;
+    a.dval = p->d_dval;
     a.sval = p->d_sval; a.sidx = p->d_sidx; a.srowptr = p->d_srowptr;
     a.xmat = nullptr; a.ymat = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
     a.tile_clock = p->clock_on ? p->d_clock : nullptr;
@@ -495,13 +503,15 @@ int sable_set_cache_dir(const char *dir)
 
 // Compiles (and caches) one kernel specialisation without a device: used at
 // build time so the code objects ship with the library.  No counterpart in
-// the reference (build tooling).  kind: 0 spmv, 1 spmm f32, 2 spmm mfma.
+// the reference (build tooling).  kind: 0 spmv, 1 spmm f32, 2 spmm mfma (split 3),
+// 3 spmm mfma (split 2).
 int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int rtmax,
                          const char *arch)
 {
     if (arch && *arch) g_arch = arch;
     Spec s;
     s.wg = wg; s.unroll = unroll; s.xcap = xcap; s.nrhs = nrhs; s.rtmax = rtmax;
+    s.split = kind == 3 ? 2 : 3;
     const char *stub = kind == 0 ? kStubSpmv : (kind == 1 ? kStubSpmmF32 : kStubSpmmMfma);
     std::vector<char> code;
     std::string path;
@@ -538,8 +548,6 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     CHECKRC(upload(&p->d_pieces, h.pieces, bytes));
     CHECKRC(upload(&p->d_xr, h.xr, bytes));
     CHECKRC(upload(&p->d_dval, h.dval, bytes));
-    CHECKRC(upload(&p->d_dhi, h.dhi, bytes));
-    CHECKRC(upload(&p->d_dlo, h.dlo, bytes));
     CHECKRC(upload(&p->d_sval, h.sval, bytes));
     CHECKRC(upload(&p->d_sidx, h.sidx, bytes));
     CHECKRC(upload(&p->d_srowptr, h.srowptr, bytes));
@@ -554,8 +562,6 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     p->device_bytes = bytes;
     // the big arrays live on the device now; keep only descriptors on the host
     p->host.dval = std::vector<float>();
-    p->host.dhi = std::vector<uint16_t>();
-    p->host.dlo = std::vector<uint16_t>();
     p->host.sval = std::vector<float>();
     p->host.sidx = std::vector<uint16_t>();
     p->host.srowptr = std::vector<int32_t>();
@@ -568,9 +574,8 @@ int sable_plan_destroy(SablePlan *p)
 {
     if (!p) return SABLE_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
-    void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_dhi, p->d_dlo, p->d_sval,
-                    p->d_sidx, p->d_srowptr, p->d_queue, p->d_xptr, p->d_yptr, p->d_clock,
-                    p->d_xtmp, p->d_ytmp, p->d_flush};
+    void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr,
+                    p->d_queue, p->d_xptr, p->d_yptr, p->d_clock, p->d_xtmp, p->d_ytmp, p->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
@@ -606,12 +611,10 @@ int sable_plan_export(const SablePlan *p, int which, void *dst, int64_t *bytes)
     case 0: hsrc = h.tiles.data(); n = h.tiles.size() * sizeof(SbTile); break;
     case 1: hsrc = h.pieces.data(); n = h.pieces.size() * sizeof(SbPiece); break;
     case 2: hsrc = h.xr.data(); n = h.xr.size() * sizeof(SbXRange); break;
-    case 3: hsrc = h.dval.data(); dsrc = p->d_dval; n = (p->keep_host ? h.dval.size() : p->stats.dense_padded) * 4; break;
+    case 3: hsrc = h.dval.data(); dsrc = p->d_dval; n = (p->stats.dense_padded + p->stats.mfma_padded) * 4; break;
     case 4: hsrc = h.sval.data(); dsrc = p->d_sval; n = p->stats.sparse_nnz * 4; break;
     case 5: hsrc = h.sidx.data(); dsrc = p->d_sidx; n = p->stats.sparse_nnz * 2; break;
     case 6: hsrc = h.srowptr.data(); dsrc = p->d_srowptr; n = p->stats.rowptr_entries * 4; break;
-    case 7: hsrc = h.dhi.data(); dsrc = p->d_dhi; n = p->stats.mfma_padded * 2; break;
-    case 8: hsrc = h.dlo.data(); dsrc = p->d_dlo; n = p->stats.mfma_padded * 2; break;
     default: return fail(SABLE_ERR_INVALID, "unknown export id");
     }
     *bytes = n;
@@ -740,7 +743,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     std::vector<SbPiece> pieces;
     std::vector<SbXRange> xr;
     std::vector<double> cost;
-    int64_t dval_n = 0, dhi_n = 0, sval_n = 0, srow_n = 0;
+    int64_t dval_n = 0, sval_n = 0, srow_n = 0;
     int maxcols = 1, wg = plans[0]->opts.wg;
     bool mfma = false;
     for (int i = 0; i < nplans; ++i) {
@@ -752,9 +755,13 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
         for (auto pc : p->host.pieces) {
             pc.xr_begin += xbase;
             pc.xr_end += xbase;
-            if (pc.kind == SB_PIECE_DENSE) pc.data_off += dval_n;
-            else if (pc.kind == SB_PIECE_DENSE_BF16) { pc.data_off += dhi_n; mfma = true; }
-            else { pc.data_off += sval_n; pc.rowptr_off += (int32_t)srow_n; }
+            if (pc.kind == SB_PIECE_DENSE || pc.kind == SB_PIECE_MFMA) {
+                pc.data_off += dval_n;
+                mfma = mfma || pc.kind == SB_PIECE_MFMA;
+            } else {
+                pc.data_off += sval_n;
+                pc.rowptr_off += (int32_t)srow_n;
+            }
             maxcols = std::max(maxcols, pc.ncols);
             pieces.push_back(pc);
         }
@@ -767,8 +774,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
             tiles.push_back(tt);
             cost.push_back(p->host.tile_cost[t]);
         }
-        dval_n += p->stats.dense_padded;
-        dhi_n += p->stats.mfma_padded;
+        dval_n += p->stats.dense_padded + p->stats.mfma_padded;
         sval_n += p->stats.sparse_nnz;
         srow_n += p->stats.rowptr_entries;
     }
@@ -798,9 +804,8 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
         return SABLE_OK;
     };
     using P2 = std::pair<const void *, int64_t>;
-    CHECKRC(cat(&g->d_dval, dval_n, [](SablePlan *p) { return P2(p->d_dval, p->stats.dense_padded); }, 4));
-    CHECKRC(cat(&g->d_dhi, dhi_n, [](SablePlan *p) { return P2(p->d_dhi, p->stats.mfma_padded); }, 2));
-    CHECKRC(cat(&g->d_dlo, dhi_n, [](SablePlan *p) { return P2(p->d_dlo, p->stats.mfma_padded); }, 2));
+    CHECKRC(cat(&g->d_dval, dval_n,
+                [](SablePlan *p) { return P2(p->d_dval, p->stats.dense_padded + p->stats.mfma_padded); }, 4));
     CHECKRC(cat(&g->d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_nnz); }, 4));
     CHECKRC(cat(&g->d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_nnz); }, 2));
     CHECKRC(cat(&g->d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
@@ -834,7 +839,6 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     } else {
         SbSpmmArgs a{};
         a.tiles = g->d_tiles; a.pieces = g->d_pieces; a.xr = g->d_xr; a.dval = g->d_dval;
-        a.dhi = g->d_dhi; a.dlo = g->d_dlo;
         a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr;
         a.xmat = g->d_xptr; a.ymat = g->d_yptr; a.queue = g->d_queue;
         a.ntiles = g->ntiles; a.mode = g->mode; a.n_rhs = g->n_rhs;
@@ -889,8 +893,8 @@ int sable_group_destroy(SableGroup *g)
     if (g->stream) hipStreamSynchronize(g->stream);
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
-    void *bufs[] = {g->d_tiles, g->d_pieces, g->d_xr, g->d_dval, g->d_dhi, g->d_dlo, g->d_sval,
-                    g->d_sidx, g->d_srowptr, g->d_queue, g->d_xptr, g->d_yptr, g->d_flush};
+    void *bufs[] = {g->d_tiles, g->d_pieces, g->d_xr, g->d_dval, g->d_sval, g->d_sidx, g->d_srowptr,
+                    g->d_queue, g->d_xptr, g->d_yptr, g->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (g->stream) hipStreamDestroy(g->stream);

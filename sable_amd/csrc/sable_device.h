@@ -16,7 +16,7 @@ typedef unsigned long uint64_t;
 
 #define SB_PIECE_DENSE 0        // column-major hs x ncols FP32 sub-panel
 #define SB_PIECE_SPARSE 1       // CSR over hs rows, uint16 local columns
-#define SB_PIECE_DENSE_BF16 2   // MFMA-ordered bf16 hi/lo sub-panel (SpMM)
+#define SB_PIECE_MFMA 2         // FP32 sub-panel in MFMA A-operand order (SpMM)
 
 // One unit of work: a row segment [row0, row0+nrows) of one block row of one
 // matrix.  The tile owns those y rows (y is written once, by this tile).
@@ -36,13 +36,13 @@ struct SbTile {
 struct SbPiece {
     int64_t data_off;        // dense: float offset (multiple of 4) into dval
                              // sparse: entry offset into sval/sidx
-                             // bf16: element offset (multiple of 8) into dhi/dlo
+                             // mfma: float offset (multiple of 8) into dval
     int32_t kind;            // SB_PIECE_*
     int32_t ncols;           // staged columns (x slice length)
     int32_t xr_begin;        // staging ranges xr[xr_begin, xr_end)
     int32_t xr_end;
     int32_t rowptr_off;      // sparse: srowptr[rowptr_off .. +nrows] (relative to data_off)
-    int32_t ksteps;          // bf16: number of 32-wide k steps (ncols padded)
+    int32_t ksteps;          // mfma: number of 32-wide k steps (ncols padded)
 };
 
 // A contiguous run of x (or X rows) copied into the staging buffer.

@@ -8,8 +8,8 @@
 // Work decomposition (see the plan builder): a tile is a row segment of one
 // block row; it owns those output rows and writes each exactly once, so no
 // atomics touch y/Y and results are deterministic.  A tile's pieces are the
-// column groups of its block row (dense FP32, dense bf16 hi/lo, or sparse
-// CSR), each with an x slice of at most SB_XCAP columns staged in LDS.
+// column groups of its block row (dense FP32, dense FP32 in MFMA order, or
+// sparse CSR), each with an x slice of at most SB_XCAP columns staged in LDS.
 #pragma once
 
 #include "sable_device.h"
@@ -56,9 +56,7 @@ struct SbSpmmArgs {
     const SbTile *tiles;
     const SbPiece *pieces;
     const SbXRange *xr;
-    const float *dval;            // FP32 dense pieces (column-major)
-    const uint16_t *dhi;          // bf16 hi parts (MFMA order)
-    const uint16_t *dlo;          // bf16 lo parts (MFMA order)
+    const float *dval;            // FP32 dense pieces (column-major) and MFMA pieces (register order)
     const float *sval;
     const uint16_t *sidx;
     const int32_t *srowptr;

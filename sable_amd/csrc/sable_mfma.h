@@ -1,19 +1,25 @@
-// SpMM with dense blocks on MFMA: bf16 hi+lo split, FP32 accumulation.
+// SpMM with dense blocks on MFMA: bf16 splits, FP32 accumulation.
 //
-// Every value a is split as a = hi + lo with hi = bf16(a), lo = bf16(a - hi)
-// (round to nearest even), likewise every X element, and the product is
-// formed from three v_mfma_f32_16x16x32_bf16 passes:
-//     A X ~= Ahi Xhi + Ahi Xlo + Alo Xhi        (Alo Xlo dropped)
-// Each bf16 product is exact in FP32; the representation error of hi+lo is
-// at most 2^-18 relative, so the result has FP32-level accuracy.
+// Every FP32 value a is split in registers into bf16 terms by round to
+// nearest even: a1 = bf16(a), a2 = bf16(a - a1), a3 = bf16(a - a1 - a2).
+// Each residual is exact in FP32, and a - a1 - a2 has at most 8 significant
+// bits, so a1 + a2 + a3 == a exactly (for normal numbers).  X is split the
+// same way while it is staged.  The product is formed from
+// v_mfma_f32_16x16x32_bf16 passes over the term pairs whose order is above
+// 2^-24 (every bf16 x bf16 product is exact in FP32):
+//   SPLIT = 3 (6 passes): a1x1 + a1x2 + a2x1 + a1x3 + a2x2 + a3x1
+//                         -> dropped terms ~2^-24 relative: FP32-level
+//   SPLIT = 2 (3 passes): a1x1 + a1x2 + a2x1 (the hi+lo scheme)
+//                         -> worst case ~3 * 2^-16 relative per product
 //
-// A (the matrix) is pre-split by the plan builder into MFMA register order:
-// for 16-row tile m and 32-column k-step s, lane l holds A[16m + l%16]
-// [32s + 8(l/16) + j], j = 0..7, as 8 consecutive bf16 (one 16-byte load).
-// X is split while it is staged: for each k-step the 32 X rows of the piece
-// are read (coalesced along N), converted, and written to LDS as [n][k] with
-// a 40-element pitch so that the B-operand reads (lane l: n = l%16,
-// k = 8(l/16) .. +7) are conflict-free 16-byte LDS reads.
+// A (the matrix) is stored by the plan builder as FP32 in MFMA register
+// order: for 16-row tile m and 32-column k-step s, lane l holds A[16m + l%16]
+// [32s + 8(l/16) + j], j = 0..7, as 8 consecutive floats (two 16-byte
+// loads).  Storing FP32 keeps the A stream at 4 bytes per value for either
+// split.  X is staged per k-step: the 32 X rows of the piece are read
+// (coalesced along N), split, and written to LDS as [n][k] term arrays with a
+// 40-element pitch so the B-operand reads (lane l: n = l%16, k = 8(l/16) ..
+// +7) are conflict-free 16-byte LDS reads.
 //
 // A tile may also hold FP32-dense and sparse pieces (other blocks of the
 // same block row); those run on the vector ALUs with the FP32 kernel's
@@ -28,6 +34,10 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 #define SB_KP 40            // LDS pitch of a staged X column (bf16 elements)
 #define SB_MFMA_MAXROWS 64  // tile height limit of the MFMA kernel
 
+#ifndef SB_SPLIT
+#define SB_SPLIT 3
+#endif
+
 __device__ __forceinline__ uint32_t sb_bf16_bits(float f)
 {
     uint32_t u = __float_as_uint(f);
@@ -40,50 +50,79 @@ __device__ __forceinline__ float sb_bf16_float(uint32_t h) { return __uint_as_fl
 
 __device__ __forceinline__ sb_bf16x8 sb_as_bf16x8(sb_u4 v) { return __builtin_bit_cast(sb_bf16x8, v); }
 
-// Stages k-step ks of a bf16 piece: Bhi/Blo[n * SB_KP + k] for k < 32.
-template <int WG>
+// Splits 8 floats into SPLIT bf16x8 terms.
+template <int SPLIT>
+__device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
+{
+    uint32_t w[SPLIT][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float ra = f[2 * j], rb = f[2 * j + 1];
+#pragma unroll
+        for (int s = 0; s < SPLIT; ++s) {
+            const uint32_t ha = sb_bf16_bits(ra), hb = sb_bf16_bits(rb);
+            w[s][j] = ha | (hb << 16);
+            ra -= sb_bf16_float(ha);
+            rb -= sb_bf16_float(hb);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SPLIT; ++s) t[s] = sb_as_bf16x8(sb_u4{w[s][0], w[s][1], w[s][2], w[s][3]});
+}
+
+// Stages k-step ks of an MFMA piece: B[s][n * SB_KP + k], k < 32, s < SPLIT.
+template <int WG, int SPLIT>
 __device__ __forceinline__ void sb_stage_b(const float *__restrict__ X, int N, const int *xrow, int ncols,
-                                           int ks, uint16_t *Bhi, uint16_t *Blo)
+                                           int ks, uint16_t *B, int term_stride)
 {
     const int units = N * 4;
     for (int u = threadIdx.x; u < units; u += WG) {
         const int kg = u / N;
         const int n = u - kg * N;
         const int k0 = ks * 32 + kg * 8;
-        uint32_t hw[4], lw[4];
+        float f[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int ka = k0 + 2 * j, kb = ka + 1;
-            const float fa = ka < ncols ? X[(size_t)xrow[ka] * N + n] : 0.f;
-            const float fb = kb < ncols ? X[(size_t)xrow[kb] * N + n] : 0.f;
-            const uint32_t ha = sb_bf16_bits(fa), hb = sb_bf16_bits(fb);
-            const uint32_t la = sb_bf16_bits(fa - sb_bf16_float(ha));
-            const uint32_t lb = sb_bf16_bits(fb - sb_bf16_float(hb));
-            hw[j] = ha | (hb << 16);
-            lw[j] = la | (lb << 16);
-        }
-        *reinterpret_cast<sb_u4 *>(Bhi + n * SB_KP + kg * 8) = sb_u4{hw[0], hw[1], hw[2], hw[3]};
-        *reinterpret_cast<sb_u4 *>(Blo + n * SB_KP + kg * 8) = sb_u4{lw[0], lw[1], lw[2], lw[3]};
+        for (int j = 0; j < 8; ++j) f[j] = (k0 + j < ncols) ? X[(size_t)xrow[k0 + j] * N + n] : 0.f;
+        sb_bf16x8 t[SPLIT];
+        sb_split8<SPLIT>(f, t);
+#pragma unroll
+        for (int s = 0; s < SPLIT; ++s)
+            *reinterpret_cast<sb_bf16x8 *>(B + s * term_stride + n * SB_KP + kg * 8) = t[s];
     }
 }
 
-template <int WG, int NRHS>
+template <int SPLIT>
+__device__ __forceinline__ sb_f32x4 sb_mfma_terms(const sb_bf16x8 *a, const sb_bf16x8 *b, sb_f32x4 acc)
+{
+    // largest-order terms last so the small ones accumulate first
+    if (SPLIT == 3) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+    return acc;
+}
+
+template <int WG, int NRHS, int SPLIT>
 struct SbMfmaCfg {
     static constexpr int NMAX = NRHS ? NRHS : 256;
     static constexpr int NTMAX = NMAX / 16;
     // vector-path rows per thread for a 64-row tile: ceil(64 * (N/4) / WG)
     static constexpr int RT0 = (SB_MFMA_MAXROWS * (NMAX / 4) + WG - 1) / WG;
     static constexpr int RTMAX = RT0 < 1 ? 1 : RT0;
-    // LDS: max(B staging hi+lo, Y tile)
-    static constexpr int BSTAGE_BYTES = 2 * NMAX * SB_KP * 2;
+    static constexpr int TERM_STRIDE = NMAX * SB_KP;            // bf16 elements per staged term
+    static constexpr int BSTAGE_BYTES = SPLIT * TERM_STRIDE * 2;
     static constexpr int YTILE_BYTES = SB_MFMA_MAXROWS * NMAX * 4;
     static constexpr int SMEM_BYTES = BSTAGE_BYTES > YTILE_BYTES ? BSTAGE_BYTES : YTILE_BYTES;
 };
 
-template <int WG, int NRHS>
+template <int WG, int NRHS, int SPLIT>
 __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem)
 {
-    using Cfg = SbMfmaCfg<WG, NRHS>;
+    using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
     sb_clock(a.tile_clock, t, 0);
     const int N = NRHS ? NRHS : a.n_rhs;
     const int nt = N >> 4;
@@ -102,8 +141,7 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     const int j0 = gi * ntw;
     const float *__restrict__ X = a.xmat ? a.xmat[tile.mat] : a.x0;
     float *__restrict__ Y = a.ymat ? a.ymat[tile.mat] : a.y0;
-    uint16_t *Bhi = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *Blo = Bhi + Cfg::NMAX * SB_KP;
+    uint16_t *B = reinterpret_cast<uint16_t *>(smem);
     float *ytile = reinterpret_cast<float *>(smem);
 
     sb_f32x4 acc[Cfg::NTMAX];
@@ -120,31 +158,32 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
         __syncthreads();
         sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
         __syncthreads();
-        if (pc.kind == SB_PIECE_DENSE_BF16) {
+        if (pc.kind == SB_PIECE_MFMA) {
             has_mfma = true;
-            const sb_u4 *__restrict__ Ahi = reinterpret_cast<const sb_u4 *>(a.dhi + pc.data_off);
-            const sb_u4 *__restrict__ Alo = reinterpret_cast<const sb_u4 *>(a.dlo + pc.data_off);
+            const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
             for (int ks = 0; ks < pc.ksteps; ++ks) {
-                sb_u4 ah = {0u, 0u, 0u, 0u}, al = {0u, 0u, 0u, 0u};
+                sb_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
                 if (mactive) {
-                    const int off = (mi * pc.ksteps + ks) * 64 + lane;
-                    ah = __builtin_nontemporal_load(&Ahi[off]);
-                    al = __builtin_nontemporal_load(&Alo[off]);
+                    const int off = ((mi * pc.ksteps + ks) * 64 + lane) * 2;
+                    a0 = __builtin_nontemporal_load(&A[off]);
+                    a1 = __builtin_nontemporal_load(&A[off + 1]);
                 }
-                sb_stage_b<WG>(X, N, xrow, pc.ncols, ks, Bhi, Blo);
+                sb_stage_b<WG, SPLIT>(X, N, xrow, pc.ncols, ks, B, Cfg::TERM_STRIDE);
                 __syncthreads();
                 if (mactive) {
-                    const sb_bf16x8 ahv = sb_as_bf16x8(ah), alv = sb_as_bf16x8(al);
+                    const float af[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                    sb_bf16x8 at[SPLIT];
+                    sb_split8<SPLIT>(af, at);
                     const int nb = lane & 15, kb = (lane >> 4) * 8;
 #pragma unroll
                     for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
                         if (jj < ntw) {
                             const int n = (j0 + jj) * 16 + nb;
-                            const sb_bf16x8 bh = sb_as_bf16x8(*reinterpret_cast<const sb_u4 *>(Bhi + n * SB_KP + kb));
-                            const sb_bf16x8 bl = sb_as_bf16x8(*reinterpret_cast<const sb_u4 *>(Blo + n * SB_KP + kb));
-                            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bh, acc[jj], 0, 0, 0);
-                            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bl, acc[jj], 0, 0, 0);
-                            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bh, acc[jj], 0, 0, 0);
+                            sb_bf16x8 bt[SPLIT];
+#pragma unroll
+                            for (int s = 0; s < SPLIT; ++s)
+                                bt[s] = *reinterpret_cast<const sb_bf16x8 *>(B + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
+                            acc[jj] = sb_mfma_terms<SPLIT>(at, bt, acc[jj]);
                         }
                     }
                 }
@@ -214,13 +253,13 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     }
 }
 
-template <int WG, int NRHS, int XCAP>
+template <int WG, int NRHS, int XCAP, int SPLIT>
 __device__ __forceinline__ void sb_spmm_mfma_body(const SbSpmmArgs &a)
 {
-    using Cfg = SbMfmaCfg<WG, NRHS>;
+    using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
     __shared__ int xrow[XCAP + 4];
     __shared__ __attribute__((aligned(16))) unsigned char smem[Cfg::SMEM_BYTES];
     __shared__ int next_tile;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmm_mfma_tile<WG, NRHS>(a, t, xrow, smem); });
+                 [&](int t) { sb_spmm_mfma_tile<WG, NRHS, SPLIT>(a, t, xrow, smem); });
 }

@@ -142,7 +142,7 @@ class Plan:
     def __init__(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, routes=None, *,
                  op="spmv", n_rhs=1, host_only=False, wg=256, xcap=4095, tile_bytes=0,
                  max_seg_rows=0, min_seg_rows=0, sp_lanes_log2=-1, mode=_lib.MODE_QUEUE,
-                 grid=0, unroll=4, use_rtc=True):
+                 grid=0, unroll=4, use_rtc=True, mfma_split=3):
         if not host_only:
             init()
         s, keep = _vbr_struct(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre)
@@ -153,6 +153,7 @@ class Plan:
         o.max_seg_rows, o.min_seg_rows, o.sp_lanes_log2 = max_seg_rows, min_seg_rows, sp_lanes_log2
         o.mode, o.grid, o.unroll = mode, grid, unroll
         o.use_rtc, o.host_only = int(bool(use_rtc)), int(bool(host_only))
+        o.mfma_split = int(mfma_split)
         r = None
         if routes is not None:
             r = np.ascontiguousarray(routes, dtype=np.uint8)
@@ -189,7 +190,7 @@ class Plan:
     # -- inspection -----------------------------------------------------
     _EXPORT = {"tiles": (0, None), "pieces": (1, None), "xranges": (2, np.int32),
                "dval": (3, np.float32), "sval": (4, np.float32), "sidx": (5, np.uint16),
-               "srowptr": (6, np.int32), "dhi": (7, np.uint16), "dlo": (8, np.uint16)}
+               "srowptr": (6, np.int32)}
     TILE_DTYPE = np.dtype([("row0", "<i4"), ("nrows", "<i4"), ("piece_begin", "<i4"),
                            ("piece_end", "<i4"), ("mat", "<i4"), ("dense_tpar", "<i4"),
                            ("dense_mcols", "<i4"), ("sp_lanes_log2", "<i4")])

@@ -1,6 +1,8 @@
-"""GPU parity of the SpMM paths (FP32 vector path and bf16 hi+lo MFMA path)
-against the FP64 oracle and the restated reference SpMM program (marked gpu).
-Bar: componentwise-scaled error <= 1e-5 for both paths."""
+"""GPU parity of the SpMM paths (FP32 vector path and the bf16-split MFMA
+path) against the FP64 oracle and the restated reference SpMM program
+(marked gpu).  Bar: componentwise-scaled error <= 1e-5 for the FP32 path and
+the exact 3-term split (6 passes); the 2-term hi+lo split (3 passes) is held
+to its derived worst case 3 * 2^-16 per product."""
 import gzip
 import os
 
@@ -137,3 +139,23 @@ def test_spmm_device_arrays_and_modes(rt):
         t = plan.time(Xd, Yd, warmup=1, reps=3)
         assert np.all(t > 0)
     assert _err(vbr, X, ref)["componentwise"] <= TOL
+
+
+def test_split_accuracy_bounds(rt):
+    """Exact 3-term split (default): FP32-level, within 1e-5 and no worse
+    than a few times the FP32 vector path; hi+lo split: within its worst-case
+    bound 3 * 2^-16 and markedly less accurate than the exact split."""
+    vbr = _c3_like(3000, seed=17)
+    rng = np.random.default_rng(9)
+    X = rng.uniform(-1, 1, (vbr.cpntr[-1], 64)).astype(np.float32)
+    routes = rng.choice([ROUTE_SPARSE, ROUTE_DENSE_MFMA], len(vbr.bindx)).astype(np.uint8)
+    e = {}
+    for split in (2, 3):
+        print(f"case: split={split}", flush=True)
+        Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=64, mfma_split=split).spmm(X)
+        e[split] = _err(vbr, X, Y)["componentwise"]
+    fp32 = np.where(routes == ROUTE_DENSE_MFMA, ROUTE_DENSE, routes)
+    e["fp32"] = _err(vbr, X, rt.Plan.from_vbr(vbr, fp32, op="spmm", n_rhs=64).spmm(X))["componentwise"]
+    print(f"  errors {e}", flush=True)
+    assert e[3] <= TOL and e[3] <= 4 * e["fp32"] + 1e-7
+    assert e[2] <= 3 * 2.0 ** -16

@@ -35,8 +35,6 @@ def reconstruct(plan):
     sval = plan.export("sval")
     sidx = plan.export("sidx")
     srow = plan.export("srowptr")
-    dhi = plan.export("dhi")
-    dlo = plan.export("dlo")
     M = np.zeros((plan.nrows, plan.ncols), dtype=np.float64)
     Mb = np.zeros_like(M)   # bf16 hi+lo parts
     cover = np.zeros(plan.nrows, dtype=np.int64)
@@ -59,17 +57,20 @@ def reconstruct(plan):
                     assert np.all(np.diff(lc) > 0)  # ascending columns in a row
                     M[r0 + r, cols[lc]] += sval[e]
             else:
+                assert off % 8 == 0
                 ks = int(p["ksteps"])
                 assert ks == (len(cols) + 31) // 32
                 mt = (hs + 15) // 16
+                n = mt * ks * 64 * 8
+                seen = np.zeros(n, dtype=bool)
                 for r in range(hs):
                     for c in range(len(cols)):
                         lane = r % 16 + 16 * ((c % 32) // 8)
-                        o = off + (((r // 16) * ks + c // 32) * 64 + lane) * 8 + c % 8
-                        Mb[r0 + r, cols[c]] += float(bf16_to_f32(dhi[o:o + 1])[0]) + float(bf16_to_f32(dlo[o:o + 1])[0])
-                # padding stays zero
-                n = mt * ks * 64 * 8
-                assert off + n <= len(dhi)
+                        o = (((r // 16) * ks + c // 32) * 64 + lane) * 8 + c % 8
+                        Mb[r0 + r, cols[c]] += dval[off + o]
+                        seen[o] = True
+                # every slot not holding a value is zero padding
+                assert np.all(dval[off:off + n][~seen] == 0)
     return M, Mb, cover
 
 
@@ -138,12 +139,9 @@ def test_host_plan_mfma_layout():
     M, Mb, cover = reconstruct(plan)
     D = vbr_to_dense(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
     assert np.all(cover == 1)
-    tot = M + Mb
-    assert np.allclose(tot, D, rtol=2 ** -15, atol=0)
+    assert np.array_equal(M + Mb, D.astype(np.float32).astype(np.float64))
+    assert np.count_nonzero(Mb) > 0
     assert all(t["nrows"] <= 64 for t in plan.export("tiles"))
-    # hi parts are the round-to-nearest-even bf16 of the values
-    dhi = plan.export("dhi")
-    assert dhi.dtype == np.uint16 and len(dhi) == plan.stats["mfma_padded"]
 
 
 def test_bad_vbr_rejected():
