@@ -76,7 +76,7 @@ typedef struct {
     int64_t tile_bytes;       /* target algorithmic bytes per tile */
     int32_t max_seg_rows;     /* upper bound on tile height (<= wg) */
     int32_t min_seg_rows;     /* lower bound on tile height when splitting */
-    int32_t sp_lanes_log2;    /* lanes per sparse row (log2); -1 = per tile */
+    int32_t sp_chunk;         /* SpMV sparse entries per LDS product pass (<= 1024; 0 = 1024) */
     int32_t mode;             /* SABLE_MODE_*; -1 = default (queue) */
     int32_t grid;             /* persistent grid size; 0 = 8 per CU */
     int32_t unroll;           /* SpMV dense stream unroll (1,2,4,8); 0 = 4 */
@@ -128,6 +128,10 @@ int sable_plan_set_stream(SablePlan *plan, void *hip_stream);
  * 6 srowptr.
  * Returns the byte size in *bytes; copies when dst != NULL. */
 int sable_plan_export(const SablePlan *plan, int which, void *dst, int64_t *bytes);
+/* The kernel specialisation chosen for the plan (no reference counterpart):
+ * spec6 = {kind, wg, unroll, xcap, nrhs, rtmax}, kind as in
+ * sable_rtc_precompile. */
+int sable_plan_kernel_spec(const SablePlan *plan, int32_t *spec6);
 
 /* Replaces: one execution of the generated program's body,
  * src/codegen.py:416-437 (memset y; CSR part; every dense block call).

@@ -47,7 +47,7 @@ class SableVbr(C.Structure):
 
 class SablePlanOpts(C.Structure):
     _fields_ = [("op", I32), ("n_rhs", I32), ("wg", I32), ("xcap", I32), ("tile_bytes", I64),
-                ("max_seg_rows", I32), ("min_seg_rows", I32), ("sp_lanes_log2", I32),
+                ("max_seg_rows", I32), ("min_seg_rows", I32), ("sp_chunk", I32),
                 ("mode", I32), ("grid", I32), ("unroll", I32), ("use_rtc", I32),
                 ("host_only", I32), ("mfma_split", I32)]
 
@@ -89,6 +89,7 @@ _SIGS = {
     "sable_plan_stats": (I32, [P, C.POINTER(SablePlanStats)]),
     "sable_plan_set_stream": (I32, [P, P]),
     "sable_plan_export": (I32, [P, I32, P, C.POINTER(I64)]),
+    "sable_plan_kernel_spec": (I32, [P, C.POINTER(I32)]),
     "sable_spmv": (I32, [P, P, P, I32]),
     "sable_spmm": (I32, [P, P, P, I32]),
     "sable_time": (I32, [P, P, P, I32, I32, I64, F32P]),

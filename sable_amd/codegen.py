@@ -43,28 +43,10 @@ DENSE_DIR = os.path.join(BASE_PATH, "Generated_dense_tensors")
 KIND_SPMV, KIND_SPMM_F32, KIND_SPMM_MFMA, KIND_SPMM_MFMA2 = 0, 1, 2, 3
 
 
-def _pow2_at_least(v, lo=16):
-    p = lo
-    while p < v:
-        p <<= 1
-    return p
-
-
-def kernel_spec(plan_host, op, n_rhs, wg=256, unroll=4, mfma_split=3):
-    """The hipRTC specialisation the runtime selects for a plan: (kind, wg,
-    unroll, xcap, nrhs, rtmax), mirroring select_kernel in runtime.hip."""
-    pieces = plan_host.export("pieces")
-    tiles = plan_host.export("tiles")
-    maxcols = int(pieces["ncols"].max()) if len(pieces) else 1
-    xcap = min(4096, _pow2_at_least(maxcols + 1))
-    if op == "spmv":
-        return (KIND_SPMV, wg, unroll, xcap, 0, 8)
-    mfma = bool(len(pieces)) and bool(np.any(pieces["kind"] == 2))
-    rows = int(tiles["nrows"].max()) if len(tiles) else 1
-    ngroups = max(1, wg // (n_rhs // 4))
-    rtmax = max(1, -(-rows // ngroups))
-    kind = (KIND_SPMM_MFMA2 if mfma_split == 2 else KIND_SPMM_MFMA) if mfma else KIND_SPMM_F32
-    return (kind, wg, unroll, xcap, n_rhs, rtmax)
+def kernel_spec(plan_host):
+    """The hipRTC specialisation the runtime selects for a plan:
+    (kind, wg, unroll, xcap, nrhs, rtmax)."""
+    return plan_host.kernel_spec()
 
 
 def _kernel_source(spec):
@@ -96,7 +78,7 @@ def _emit(dir_name, filename, arrays, meta, op, n_rhs, routes):
              **arrays)
     host = Plan(arrays["val"], arrays["indx"], arrays["bindx"], arrays["rpntr"], arrays["cpntr"],
                 arrays["bpntrb"], arrays["bpntre"], routes, op=op, n_rhs=n_rhs, host_only=True, mfma_split=split)
-    spec = kernel_spec(host, op, n_rhs, mfma_split=split)
+    spec = kernel_spec(host)
     with open(os.path.join(dir_name, filename + ".hip"), "w") as f:
         f.write(_kernel_source(spec))
     _compile(spec)

@@ -35,8 +35,7 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 48 * 1024 : 64 * 1024;
     if (o.max_seg_rows <= 0) o.max_seg_rows = o.wg;
     if (o.min_seg_rows <= 0) o.min_seg_rows = (o.op == SABLE_OP_SPMV) ? 8 : 16;
-    if (o.sp_lanes_log2 > 6) o.sp_lanes_log2 = 6;
-    if (o.sp_lanes_log2 < -1) o.sp_lanes_log2 = -1;
+    if (o.sp_chunk <= 0 || o.sp_chunk > 1024) o.sp_chunk = 1024;
     if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_QUEUE;
     if (o.unroll <= 0) o.unroll = 4;
     if (o.mfma_split != 2) o.mfma_split = 3;
@@ -125,13 +124,6 @@ void push_xranges(HostPlan &P, const std::vector<ColSeg> &g, SbPiece &pc)
         P.xr.push_back({s.gcol, s.n});
     }
     pc.xr_end = (int32_t)P.xr.size();
-}
-
-int ilog2_ceil(int64_t v)
-{
-    int l = 0;
-    while ((int64_t(1) << l) < v) ++l;
-    return l;
 }
 
 void dense_stream_params(int hs, int wg, int32_t &tpar, int32_t &mcols)
@@ -351,9 +343,7 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                 cost += 6.0 * tot + 4.0 * (hs + 1);
             }
             t.piece_end = (int32_t)P.pieces.size();
-            const int64_t avg = (seg_sp + hs - 1) / std::max(hs, 1);
-            int lg = o.sp_lanes_log2 >= 0 ? o.sp_lanes_log2 : ilog2_ceil(std::max<int64_t>(1, avg / 2));
-            t.sp_lanes_log2 = std::min(lg, 6);
+            t.sp_chunk = o.sp_chunk;
             if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
             P.tiles.push_back(t);
             P.tile_cost.push_back(cost);

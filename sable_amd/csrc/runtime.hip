@@ -282,6 +282,7 @@ struct SablePlan {
     const void *aot = nullptr;
     int grid = 0;
     int64_t device_bytes = 0;
+    int32_t spec[6] = {0, 0, 0, 0, 0, 0};   // kind, wg, unroll, xcap, nrhs, rtmax
 };
 
 struct SableGroup {
@@ -334,6 +335,19 @@ int max_piece_cols(const sable::HostPlan &h)
     return m;
 }
 
+// LDS x buffer a SpMV tile needs to stage all its pieces at once
+// (each slice followed by 4 zero pads).
+int max_tile_staging(const sable::HostPlan &h)
+{
+    int m = 1;
+    for (auto &t : h.tiles) {
+        int s = 0;
+        for (int p = t.piece_begin; p < t.piece_end; ++p) s += h.pieces[p].ncols + 4;
+        m = std::max(m, s);
+    }
+    return m;
+}
+
 bool has_kind(const sable::HostPlan &h, int kind)
 {
     for (auto &p : h.pieces)
@@ -343,15 +357,19 @@ bool has_kind(const sable::HostPlan &h, int kind)
 
 // chooses the kernel for a plan (RTC specialisation or AOT generic)
 int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, hipFunction_t &fn,
-                  const void *&aot)
+                  const void *&aot, int32_t spec_out[6])
 {
     rtc = false;
     fn = nullptr;
     Spec s;
     s.wg = o.wg;
     s.unroll = o.unroll;
-    s.xcap = std::min(4096, pow2_at_least(max_piece_cols(h) + 1));
     const bool spmm = h.op == SABLE_OP_SPMM;
+    // SpMV: room for a whole tile's slices when possible; never below the
+    // widest piece (the kernel needs every piece to fit on its own)
+    s.xcap = spmm ? std::min(4096, pow2_at_least(max_piece_cols(h) + 1))
+                  : std::min(4096, std::max(pow2_at_least(max_tile_staging(h)),
+                                            pow2_at_least(max_piece_cols(h))));
     const bool mfma = spmm && has_kind(h, SB_PIECE_MFMA);
     s.split = o.mfma_split == 2 ? 2 : 3;
     if (spmm) {
@@ -365,6 +383,12 @@ int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, h
         aot = mfma ? (s.split == 2 ? (const void *)sb_spmm_mfma2_aot : (const void *)sb_spmm_mfma_aot)
                    : (const void *)sb_spmm_f32_aot;
     else aot = (const void *)sb_spmv_aot;
+    spec_out[0] = spmm ? (mfma ? (s.split == 2 ? 3 : 2) : 1) : 0;
+    spec_out[1] = s.wg;
+    spec_out[2] = s.unroll;
+    spec_out[3] = s.xcap;
+    spec_out[4] = s.nrhs;
+    spec_out[5] = s.rtmax;
     if (!o.use_rtc) {
         if (o.wg != 256 || (!spmm && o.unroll != 4)) return fail(SABLE_ERR_UNSUPPORTED, "AOT kernels exist only for wg=256 / unroll=4");
         return SABLE_OK;
@@ -537,11 +561,17 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     p->stats = p->host.stats;
     p->host_only = o.host_only != 0;
     if (p->host_only) {
+        SablePlanOpts no_rtc = o;
+        no_rtc.use_rtc = 0;
+        bool r;
+        hipFunction_t f;
+        const void *ao;
+        select_kernel(p->host, no_rtc, r, f, ao, p->spec);   // spec only, nothing compiled
         *out = p.release();
         return SABLE_OK;
     }
     if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
-    CHECKRC(select_kernel(p->host, o, p->rtc, p->fn, p->aot));
+    CHECKRC(select_kernel(p->host, o, p->rtc, p->fn, p->aot, p->spec));
     const sable::HostPlan &h = p->host;
     int64_t bytes = 0;
     CHECKRC(upload(&p->d_tiles, h.tiles, bytes));
@@ -580,6 +610,13 @@ int sable_plan_destroy(SablePlan *p)
         if (b) hipFree(b);
     if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
     delete p;
+    return SABLE_OK;
+}
+
+int sable_plan_kernel_spec(const SablePlan *p, int32_t *spec6)
+{
+    if (!p || !spec6) return fail(SABLE_ERR_INVALID, "null argument");
+    for (int i = 0; i < 6; ++i) spec6[i] = p->spec[i];
     return SABLE_OK;
 }
 
@@ -821,7 +858,8 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     view.n_rhs = g->n_rhs;
     view.tiles = sorted;
     view.pieces = pieces;
-    CHECKRC(select_kernel(view, plans[0]->opts, g->rtc, g->fn, g->aot));
+    int32_t spec[6];
+    CHECKRC(select_kernel(view, plans[0]->opts, g->rtc, g->fn, g->aot, spec));
     g->grid = g->mode == SABLE_MODE_GRID ? g->ntiles
                                           : occupancy_grid(g->rtc, g->fn, g->aot, wg, std::max(1, g->ntiles), grid);
     for (int i = 0; i < nplans; ++i) g->plans.push_back(plans[i]);

@@ -28,7 +28,7 @@ struct SbTile {
     int32_t mat;             // matrix slot (batched plans); 0 otherwise
     int32_t dense_tpar;      // T': active lanes of the flat dense stream
     int32_t dense_mcols;     // M : columns advanced per stream step (4T' = M*nrows)
-    int32_t sp_lanes_log2;   // lanes per sparse row = 1 << sp_lanes_log2
+    int32_t sp_chunk;        // sparse entries per product pass (<= the LDS chunk)
 };
 
 // A panel piece of a tile: the tile's rows restricted to a group of blocks

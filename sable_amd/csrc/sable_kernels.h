@@ -115,39 +115,129 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
     }
 }
 
-// Copies the x slice of a piece into xs[0..ncols) and zeroes xs[ncols..ncols+3]
-// (the flat dense stream of a 1- or 2-row tile can touch up to xs[ncols+2],
-// always multiplied by the zero padding of the panel).
-__device__ __forceinline__ void sb_stage_x(const SbXRange *__restrict__ xr, int r0, int r1,
-                                           const float *__restrict__ x, float *xs, int ncols)
-{
-    int base = 0;
-    for (int r = r0; r < r1; ++r) {
-        const SbXRange rg = xr[r];
-        for (int i = threadIdx.x; i < rg.len; i += blockDim.x) xs[base + i] = x[rg.col0 + i];
-        base += rg.len;
-    }
-    if (threadIdx.x < 4) xs[ncols + threadIdx.x] = 0.f;
-}
-
 // ---------------------------------------------------------------------------
 // SpMV
 //
+// A tile's pieces are processed in staging groups: consecutive pieces whose x
+// slices (each followed by 4 zero pads) fit in the LDS buffer are staged in
+// one flattened pass (every thread issues its loads together, so the staging
+// costs one memory round trip), then processed.
+//
 // Dense piece = column-major hs x ncols FP32 sub-panel, contiguous, 16-byte
-// aligned, zero-padded to a multiple of 4 floats.  It is streamed as float4:
-// lane t (t < T') reads float4 q = t + k*T'.  Because 4*T' = M*hs, lane t
-// always sees the same four row indices (4t+s) % hs and walks columns
-// (4t+s)/hs + k*M, so each lane accumulates four fixed rows in registers and
-// the wavefront's loads are fully coalesced 16-byte accesses.  The M partial
-// sums of every row are combined by a deterministic LDS tree at the end.
+// aligned, zero-padded to a multiple of 4 floats, streamed as float4: lane t
+// (t < T') reads float4 q = t + k*T'.  Because 4*T' = M*hs, lane t always
+// sees the same four row indices (4t+s) % hs and walks columns (4t+s)/hs +
+// k*M, so each lane accumulates four fixed rows in registers and every
+// wavefront load is a fully coalesced 16-byte access.  Loads are software
+// pipelined (two register batches of UNROLL float4), and the first batch of
+// a group's first dense piece is issued before the staging barrier.
 //
 // Sparse piece = CSR over the tile's rows with uint16 column indices into the
-// staged x slice; 2^k lanes per row stride over the row's entries and reduce
-// with cross-lane shuffles (k chosen per tile from the mean row length).
+// staged x slice.  Entries are read in one flat coalesced pass per chunk of
+// SB_SPCHUNK entries, their products v*x[col] written to LDS, and each row's
+// owner thread then sums its CSR range in column order.
+//
+// Row r's result is (dense partial sums of the M column groups, in group
+// order) + (sparse products in CSR order): deterministic, no atomics.
+
+#ifndef SB_SPCHUNK
+#define SB_SPCHUNK 1024
+#endif
+
+struct SbStreamLane {
+    int c0, c1, c2, c3;    // column offsets of this lane's four elements
+};
+
+template <int UNROLL>
+__device__ __forceinline__ void sb_dense_load(const sb_f4 *__restrict__ P, int nq, int q0, int tpar,
+                                              sb_f4 *v)
+{
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+        const int q = q0 + u * tpar;
+        v[u] = q < nq ? __builtin_nontemporal_load(&P[q]) : sb_f4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
+template <int UNROLL>
+__device__ __forceinline__ void sb_dense_fma(const sb_f4 *v, const float *xs, const SbStreamLane &L, int kc,
+                                             int M, float &a0, float &a1, float &a2, float &a3)
+{
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+        const int k = kc + u * M;
+        a0 += v[u].x * xs[L.c0 + k];
+        a1 += v[u].y * xs[L.c1 + k];
+        a2 += v[u].z * xs[L.c2 + k];
+        a3 += v[u].w * xs[L.c3 + k];
+    }
+}
+
+// Streams one dense piece whose x slice starts at xs (batch 0 preloaded in cur
+// when `preloaded`).
+template <int UNROLL>
+__device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int nq, int tpar, int M,
+                                               const float *xs, const SbStreamLane &L, sb_f4 *cur,
+                                               bool preloaded, float &a0, float &a1, float &a2, float &a3)
+{
+    const int tid = threadIdx.x;
+    if (tid >= tpar || tid >= nq) return;
+    const int nmine = (nq - 1 - tid) / tpar + 1;             // float4 loads of this lane
+    const int nbatch = (nmine + UNROLL - 1) / UNROLL;
+    if (!preloaded) sb_dense_load<UNROLL>(P, nq, tid, tpar, cur);
+    sb_f4 nxt[UNROLL];
+    for (int b = 0; b < nbatch; ++b) {
+        if (b + 1 < nbatch) sb_dense_load<UNROLL>(P, nq, tid + (b + 1) * UNROLL * tpar, tpar, nxt);
+        sb_dense_fma<UNROLL>(cur, xs, L, b * UNROLL * M, M, a0, a1, a2, a3);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) cur[u] = nxt[u];
+    }
+}
+
+// Stages the x slices of pieces [pb, pe) into xs: piece p at base_p, each
+// slice followed by 4 zeros.  One flattened pass, 4 loads in flight per thread.
+template <int WG>
+__device__ __forceinline__ void sb_stage_group(const SbPiece *__restrict__ pieces, const SbXRange *__restrict__ xr,
+                                               int pb, int pe, const float *__restrict__ x, float *xs)
+{
+    int total = 0;
+    for (int p = pb; p < pe; ++p) total += pieces[p].ncols + 4;
+    for (int i0 = 0; i0 < total; i0 += 4 * WG) {
+        float v[4];
+        int dst[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * WG + threadIdx.x;
+            dst[u] = -1;
+            v[u] = 0.f;
+            if (i < total) {
+                dst[u] = i;
+                int base = 0, p = pb;
+                while (i >= base + pieces[p].ncols + 4) {
+                    base += pieces[p].ncols + 4;
+                    ++p;
+                }
+                int off = i - base;
+                if (off < pieces[p].ncols) {
+                    for (int r = pieces[p].xr_begin; r < pieces[p].xr_end; ++r) {
+                        const SbXRange g = xr[r];
+                        if (off < g.len) {
+                            v[u] = x[g.col0 + off];
+                            break;
+                        }
+                        off -= g.len;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (dst[u] >= 0) xs[dst[u]] = v[u];
+    }
+}
 
 template <int WG, int UNROLL, int XCAP>
-__device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *xs, float *red,
-                                             float *ysp)
+__device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *xs, float *red, float *prod)
 {
     const int tid = threadIdx.x;
     sb_clock(a.tile_clock, t, 0);
@@ -157,81 +247,88 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
     const int M = tile.dense_mcols;
     const float *__restrict__ x = a.xvec ? a.xvec[tile.mat] : a.x0;
     float *__restrict__ y = a.yvec ? a.yvec[tile.mat] : a.y0;
+    const int chunk = tile.sp_chunk > 0 && tile.sp_chunk <= SB_SPCHUNK ? tile.sp_chunk : SB_SPCHUNK;
 
-    const int p0 = 4 * tid;
-    const int c0 = p0 / hs, c1 = (p0 + 1) / hs, c2 = (p0 + 2) / hs, c3 = (p0 + 3) / hs;
-    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
-    if (tid < hs) ysp[tid] = 0.f;
+    SbStreamLane L;
+    {
+        const int p0 = 4 * tid;
+        L.c0 = p0 / hs;
+        L.c1 = (p0 + 1) / hs;
+        L.c2 = (p0 + 2) / hs;
+        L.c3 = (p0 + 3) / hs;
+    }
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    float ysum = 0.f;
+    sb_f4 cur[UNROLL];
 
-    for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
-        const SbPiece pc = a.pieces[pi];
+    int pb = tile.piece_begin;
+    while (pb < tile.piece_end) {
+        // staging group [pb, pe)
+        int pe = pb, used = 0;
+        while (pe < tile.piece_end && used + a.pieces[pe].ncols + 4 <= XCAP + 4) used += a.pieces[pe++].ncols + 4;
+        if (pe == pb) break;   // a piece wider than the buffer: never built (the host checks)
+        // issue the first dense batch before staging (it does not need x)
+        const SbPiece first = a.pieces[pb];
+        const bool pre = first.kind == SB_PIECE_DENSE && tid < tpar;
+        if (pre) sb_dense_load<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + first.data_off),
+                                       (hs * first.ncols + 3) >> 2, tid, tpar, cur);
+        __syncthreads();                       // previous group done with xs / prod
+        sb_stage_group<WG>(a.pieces, a.xr, pb, pe, x, xs);
         __syncthreads();
-        sb_stage_x(a.xr, pc.xr_begin, pc.xr_end, x, xs, pc.ncols);
-        __syncthreads();
-        if (pc.kind == SB_PIECE_DENSE) {
-            const int nq = (hs * pc.ncols + 3) >> 2;
-            const sb_f4 *__restrict__ P = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
-            if (tid < tpar) {
-                int q = tid;
-                int kc = 0;
-                for (; q + (UNROLL - 1) * tpar < nq; q += UNROLL * tpar, kc += UNROLL * M) {
-                    sb_f4 v[UNROLL];
-#pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) v[u] = __builtin_nontemporal_load(&P[q + u * tpar]);
-#pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) {
-                        const int k = kc + u * M;
-                        acc0 += v[u].x * xs[c0 + k];
-                        acc1 += v[u].y * xs[c1 + k];
-                        acc2 += v[u].z * xs[c2 + k];
-                        acc3 += v[u].w * xs[c3 + k];
+        int base = 0;
+        for (int p = pb; p < pe; ++p) {
+            const SbPiece pc = a.pieces[p];
+            if (pc.kind == SB_PIECE_DENSE) {
+                sb_dense_piece<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off),
+                                       (hs * pc.ncols + 3) >> 2, tpar, M, xs + base, L, cur, pre && p == pb,
+                                       a0, a1, a2, a3);
+            } else {
+                const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
+                const float *__restrict__ sv = a.sval + pc.data_off;
+                const uint16_t *__restrict__ si = a.sidx + pc.data_off;
+                const int nnzp = rp[hs];
+                const int rb = tid < hs ? rp[tid] : 0;
+                const int re = tid < hs ? rp[tid + 1] : 0;
+                for (int e0 = 0; e0 < nnzp; e0 += chunk) {
+                    const int n = min(chunk, nnzp - e0);
+                    __syncthreads();           // prod reuse
+#pragma unroll 4
+                    for (int e = tid; e < n; e += WG) prod[e] = sv[e0 + e] * xs[base + si[e0 + e]];
+                    __syncthreads();
+                    if (tid < hs) {
+                        const int lo = max(rb, e0), hi = min(re, e0 + n);
+                        for (int e = lo; e < hi; ++e) ysum += prod[e - e0];
                     }
                 }
-                for (; q < nq; q += tpar, kc += M) {
-                    const sb_f4 v = __builtin_nontemporal_load(&P[q]);
-                    acc0 += v.x * xs[c0 + kc];
-                    acc1 += v.y * xs[c1 + kc];
-                    acc2 += v.z * xs[c2 + kc];
-                    acc3 += v.w * xs[c3 + kc];
-                }
             }
-        } else {
-            const int lg = tile.sp_lanes_log2;
-            const int L = 1 << lg;
-            const int g = tid >> lg;
-            const int l = tid & (L - 1);
-            const int per_pass = WG >> lg;
-            const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
-            const float *__restrict__ sv = a.sval + pc.data_off;
-            const uint16_t *__restrict__ si = a.sidx + pc.data_off;
-            for (int r = g; r < hs; r += per_pass) {
-                const int b = rp[r], e = rp[r + 1];
-                float s = 0.f;
-                for (int p = b + l; p < e; p += L) s += sv[p] * xs[si[p]];
-                for (int o = L >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
-                if (l == 0) ysp[r] += s;
-            }
+            base += pc.ncols + 4;
         }
+        pb = pe;
     }
 
     __syncthreads();
     if (tid < tpar) {
-        red[p0] = acc0;
-        red[p0 + 1] = acc1;
-        red[p0 + 2] = acc2;
-        red[p0 + 3] = acc3;
+        red[4 * tid] = a0;
+        red[4 * tid + 1] = a1;
+        red[4 * tid + 2] = a2;
+        red[4 * tid + 3] = a3;
     }
     __syncthreads();
-    // red holds M groups of hs partial sums: red[m*hs + r].  Fold halves.
+    // red holds M groups of hs partial sums: red[m*hs + r].  Fold halves while
+    // there are many groups (short tiles), then each row owner sums in order.
     int m = M;
-    while (m > 1) {
+    while (m > 32) {
         const int half = (m + 1) >> 1;
         const int cnt = (m - half) * hs;
         for (int i = tid; i < cnt; i += WG) red[i] += red[i + half * hs];
         __syncthreads();
         m = half;
     }
-    if (tid < hs) y[tile.row0 + tid] = red[tid] + ysp[tid];
+    if (tid < hs) {
+        float s = 0.f;
+        for (int g = 0; g < m; ++g) s += red[g * hs + tid];
+        y[tile.row0 + tid] = s + ysum;
+    }
     if (a.tile_clock) {
         __syncthreads();
         sb_clock(a.tile_clock, t, 1);
@@ -243,10 +340,10 @@ __device__ __forceinline__ void sb_spmv_body(const SbSpmvArgs &a)
 {
     __shared__ float xs[XCAP + 4];
     __shared__ float red[4 * WG];
-    __shared__ float ysp[WG];
+    __shared__ float prod[SB_SPCHUNK];
     __shared__ int next_tile;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmv_tile<WG, UNROLL, XCAP>(a, t, xs, red, ysp); });
+                 [&](int t) { sb_spmv_tile<WG, UNROLL, XCAP>(a, t, xs, red, prod); });
 }
 
 // ---------------------------------------------------------------------------

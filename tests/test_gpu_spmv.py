@@ -136,12 +136,14 @@ def test_edge_shapes(rt):
     assert np.all(y == 0)
 
 
-@pytest.mark.parametrize("lanes", [0, 2, 4, 6])
-def test_sparse_lane_widths(rt, lanes):
+@pytest.mark.parametrize("chunk", [1, 7, 256, 1024])
+def test_sparse_chunk_sizes(rt, chunk):
+    """Products staged through LDS in chunks: rows that straddle chunk
+    boundaries (chunk sizes 1 and 7 force it) still sum in CSR order."""
     vbr, x, _ = synth.workload_like_c2(4000, seed=13, frac_blocks=0.03)
     routes = np.full(len(vbr.bindx), ROUTE_SPARSE, np.uint8)
-    print(f"case: sparse lanes 2^{lanes}", flush=True)
-    y = rt.Plan.from_vbr(vbr, routes, sp_lanes_log2=lanes).spmv(x)
+    print(f"case: sparse chunk {chunk}", flush=True)
+    y = rt.Plan.from_vbr(vbr, routes, sp_chunk=chunk).spmv(x)
     assert _err(vbr, x, y)["componentwise"] <= TOL
 
 

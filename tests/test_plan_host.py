@@ -101,7 +101,7 @@ def test_host_plan_launch_invariants():
         hs, tp, M = int(t["nrows"]), int(t["dense_tpar"]), int(t["dense_mcols"])
         assert 1 <= hs <= 256
         assert 4 * tp == M * hs and 0 < tp <= 256
-        assert 0 <= t["sp_lanes_log2"] <= 6
+        assert 0 < t["sp_chunk"] <= 1024
     for p in plan.export("pieces"):
         assert p["ncols"] <= 4096
 
