@@ -8,8 +8,9 @@ vs a persistent work queue), and for SpMM the MFMA split.  ``tune`` builds a
 plan per candidate, checks its result against the first candidate's (all
 candidates compute the same sums in the same per-row order except for the
 tile boundaries, so results agree to rounding), times it cold, and returns
-the table.  ``TUNED`` holds the winners measured on MI355X for the workloads
-of this repository; ``tuned_options`` looks them up.
+the table.  ``tuning.json`` next to this module holds the winners measured
+on MI355X for the workloads of this repository; ``tuned_options`` looks them
+up.
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.js
 
 SPMV_SPACE = {
     "unroll": (2, 4, 8),
-    "tile_bytes": (16 << 10, 32 << 10, 64 << 10, 128 << 10),
+    "tile_bytes": (64 << 10, 128 << 10, 256 << 10, 512 << 10, 1 << 20),
     "mode": (0, 2),
     "wg": (256,),
 }

@@ -1,7 +1,9 @@
 // Host plan builder.  See plan.h.
 //
 // Layout decisions (all per block row, after routing):
-//  * dense-routed blocks of a block row are concatenated in ordinal order:
+//  * dense-routed blocks of a block row are concatenated in ordinal order
+//    (each sub-panel is followed by one zero float4 that the kernel's
+//    unconditional loads read past the end):
 //    in the VBR they already form one column-major h x W panel (same height,
 //    consecutive storage), so a row segment of that panel becomes one
 //    contiguous column-major sub-panel per column group;
@@ -254,6 +256,7 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                     }
                 }
                 while (P.dval.size() % 4) P.dval.push_back(0.f);
+                P.dval.insert(P.dval.end(), 4, 0.f);   // the zero float4 read past the end
                 S.dense_values += (int64_t)hs * pc.ncols;
                 push_xranges(P, g, pc);
                 P.pieces.push_back(pc);

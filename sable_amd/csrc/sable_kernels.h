@@ -124,7 +124,8 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
 // costs one memory round trip), then processed.
 //
 // Dense piece = column-major hs x ncols FP32 sub-panel, contiguous, 16-byte
-// aligned, zero-padded to a multiple of 4 floats, streamed as float4: lane t
+// aligned, zero-padded to a multiple of 4 floats and followed by one zero
+// float4, streamed as float4: lane t
 // (t < T') reads float4 q = t + k*T'.  Because 4*T' = M*hs, lane t always
 // sees the same four row indices (4t+s) % hs and walks columns (4t+s)/hs +
 // k*M, so each lane accumulates four fixed rows in registers and every
@@ -148,6 +149,10 @@ struct SbStreamLane {
     int c0, c1, c2, c3;    // column offsets of this lane's four elements
 };
 
+// Loads are unconditional and nothing is applied to their values until the
+// batch is consumed, so the compiler can wait for one batch while the next
+// is still in flight: an index past the end reads the zero float4 the plan
+// builder appends after every dense piece (P[nq]).
 template <int UNROLL>
 __device__ __forceinline__ void sb_dense_load(const sb_f4 *__restrict__ P, int nq, int q0, int tpar,
                                               sb_f4 *v)
@@ -155,28 +160,30 @@ __device__ __forceinline__ void sb_dense_load(const sb_f4 *__restrict__ P, int n
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
         const int q = q0 + u * tpar;
-        v[u] = q < nq ? __builtin_nontemporal_load(&P[q]) : sb_f4{0.f, 0.f, 0.f, 0.f};
+        v[u] = __builtin_nontemporal_load(&P[q < nq ? q : nq]);
     }
 }
 
+// The x index is clamped to the slice's first zero pad (xs[ncols]): past the
+// end of the stream the loaded value is zero and so is the x it meets.
 template <int UNROLL>
 __device__ __forceinline__ void sb_dense_fma(const sb_f4 *v, const float *xs, const SbStreamLane &L, int kc,
-                                             int M, float &a0, float &a1, float &a2, float &a3)
+                                             int M, int ncols, float &a0, float &a1, float &a2, float &a3)
 {
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
         const int k = kc + u * M;
-        a0 += v[u].x * xs[L.c0 + k];
-        a1 += v[u].y * xs[L.c1 + k];
-        a2 += v[u].z * xs[L.c2 + k];
-        a3 += v[u].w * xs[L.c3 + k];
+        a0 += v[u].x * xs[min(L.c0 + k, ncols)];
+        a1 += v[u].y * xs[min(L.c1 + k, ncols)];
+        a2 += v[u].z * xs[min(L.c2 + k, ncols)];
+        a3 += v[u].w * xs[min(L.c3 + k, ncols)];
     }
 }
 
 // Streams one dense piece whose x slice starts at xs (batch 0 preloaded in cur
 // when `preloaded`).
 template <int UNROLL>
-__device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int nq, int tpar, int M,
+__device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int nq, int tpar, int M, int ncols,
                                                const float *xs, const SbStreamLane &L, sb_f4 *cur,
                                                bool preloaded, float &a0, float &a1, float &a2, float &a3)
 {
@@ -187,8 +194,9 @@ __device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int 
     if (!preloaded) sb_dense_load<UNROLL>(P, nq, tid, tpar, cur);
     sb_f4 nxt[UNROLL];
     for (int b = 0; b < nbatch; ++b) {
-        if (b + 1 < nbatch) sb_dense_load<UNROLL>(P, nq, tid + (b + 1) * UNROLL * tpar, tpar, nxt);
-        sb_dense_fma<UNROLL>(cur, xs, L, b * UNROLL * M, M, a0, a1, a2, a3);
+        // always prefetch (the batch after the last one reads the zero float4)
+        sb_dense_load<UNROLL>(P, nq, tid + (b + 1) * UNROLL * tpar, tpar, nxt);
+        sb_dense_fma<UNROLL>(cur, xs, L, b * UNROLL * M, M, ncols, a0, a1, a2, a3);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) cur[u] = nxt[u];
     }
@@ -280,7 +288,7 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
             const SbPiece pc = a.pieces[p];
             if (pc.kind == SB_PIECE_DENSE) {
                 sb_dense_piece<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off),
-                                       (hs * pc.ncols + 3) >> 2, tpar, M, xs + base, L, cur, pre && p == pb,
+                                       (hs * pc.ncols + 3) >> 2, tpar, M, pc.ncols, xs + base, L, cur, pre && p == pb,
                                        a0, a1, a2, a3);
             } else {
                 const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
@@ -386,52 +394,104 @@ __device__ __forceinline__ SbVecMap sb_vec_map(int hs, int N, int wg)
     return m;
 }
 
+#ifndef SB_SPMM_SPCHUNK
+#define SB_SPMM_SPCHUNK 512
+#endif
+
+__device__ __forceinline__ void sb_fma4(float4 &acc, float a, const float4 &x)
+{
+    acc.x += a * x.x;
+    acc.y += a * x.y;
+    acc.z += a * x.z;
+    acc.w += a * x.w;
+}
+
 // Accumulates one FP32-dense or sparse piece into acc (RTMAX row registers).
-template <int RTMAX>
+// Dense: columns are processed 4 at a time with every load of the group
+// issued before its FMAs.  Sparse: the piece's entries are staged through
+// LDS (value and X row index) in chunks of SB_SPMM_SPCHUNK, then each thread
+// walks its rows' entries with 4 X-row loads in flight.  spv/spx are LDS
+// scratch of SB_SPMM_SPCHUNK entries.
+template <int WG, int RTMAX>
 __device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbPiece &pc, int hs, int N,
                                                   const SbVecMap &m, const int *xrow,
-                                                  const float *__restrict__ X, float4 *acc)
+                                                  const float *__restrict__ X, float4 *acc, float *spv, int *spx)
 {
     if (pc.kind == SB_PIECE_DENSE) {
+        if (!m.active) return;
         const float *__restrict__ P = a.dval + pc.data_off;
-        for (int c = 0; c < pc.ncols; ++c) {
+        int c = 0;
+        for (; c + 4 <= pc.ncols; c += 4) {
+            float4 xv[4];
+            float av[4][RTMAX];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                xv[u] = *reinterpret_cast<const float4 *>(X + (size_t)xrow[c + u] * N + 4 * m.v);
+                const float *__restrict__ col = P + (size_t)(c + u) * hs;
+#pragma unroll
+                for (int i = 0; i < RTMAX; ++i) {
+                    const int r = m.rbeg + i;
+                    av[u][i] = (i < m.rt && r < hs) ? col[r] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int i = 0; i < RTMAX; ++i) sb_fma4(acc[i], av[u][i], xv[u]);
+        }
+        for (; c < pc.ncols; ++c) {
             const float4 xv = *reinterpret_cast<const float4 *>(X + (size_t)xrow[c] * N + 4 * m.v);
             const float *__restrict__ col = P + (size_t)c * hs;
 #pragma unroll
             for (int i = 0; i < RTMAX; ++i) {
                 const int r = m.rbeg + i;
-                if (i < m.rt && r < hs) {
-                    const float av = col[r];
-                    acc[i].x += av * xv.x;
-                    acc[i].y += av * xv.y;
-                    acc[i].z += av * xv.z;
-                    acc[i].w += av * xv.w;
-                }
+                if (i < m.rt && r < hs) sb_fma4(acc[i], col[r], xv);
             }
         }
-    } else if (pc.kind == SB_PIECE_SPARSE) {
-        const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
-        const float *__restrict__ sv = a.sval + pc.data_off;
-        const uint16_t *__restrict__ si = a.sidx + pc.data_off;
+        return;
+    }
+    if (pc.kind != SB_PIECE_SPARSE) return;
+    const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
+    const float *__restrict__ sv = a.sval + pc.data_off;
+    const uint16_t *__restrict__ si = a.sidx + pc.data_off;
+    const int nnzp = rp[hs];
+    int rlo[RTMAX], rhi[RTMAX];
+#pragma unroll
+    for (int i = 0; i < RTMAX; ++i) {
+        const int r = m.rbeg + i;
+        const bool ok = m.active && i < m.rt && r < hs;
+        rlo[i] = ok ? rp[r] : 0;
+        rhi[i] = ok ? rp[r + 1] : 0;
+    }
+    for (int e0 = 0; e0 < nnzp; e0 += SB_SPMM_SPCHUNK) {
+        const int n = min(SB_SPMM_SPCHUNK, nnzp - e0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += WG) {
+            spv[e] = sv[e0 + e];
+            spx[e] = xrow[si[e0 + e]];
+        }
+        __syncthreads();
+        if (!m.active) continue;
 #pragma unroll
         for (int i = 0; i < RTMAX; ++i) {
-            const int r = m.rbeg + i;
-            if (i < m.rt && r < hs) {
-                for (int p = rp[r]; p < rp[r + 1]; ++p) {
-                    const float av = sv[p];
-                    const float4 xv = *reinterpret_cast<const float4 *>(X + (size_t)xrow[si[p]] * N + 4 * m.v);
-                    acc[i].x += av * xv.x;
-                    acc[i].y += av * xv.y;
-                    acc[i].z += av * xv.z;
-                    acc[i].w += av * xv.w;
-                }
+            const int lo = max(rlo[i], e0) - e0, hi = min(rhi[i], e0 + n) - e0;
+            int e = lo;
+            for (; e + 4 <= hi; e += 4) {
+                float4 xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    xv[u] = *reinterpret_cast<const float4 *>(X + (size_t)spx[e + u] * N + 4 * m.v);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sb_fma4(acc[i], spv[e + u], xv[u]);
             }
+            for (; e < hi; ++e)
+                sb_fma4(acc[i], spv[e], *reinterpret_cast<const float4 *>(X + (size_t)spx[e] * N + 4 * m.v));
         }
     }
 }
 
 template <int WG, int RTMAX, int NRHS>
-__device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int *xrow)
+__device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int *xrow, float *spv, int *spx)
 {
     sb_clock(a.tile_clock, t, 0);
     const int N = NRHS ? NRHS : a.n_rhs;
@@ -449,7 +509,7 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
         __syncthreads();
         sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
         __syncthreads();
-        if (m.active) sb_spmm_vec_piece<RTMAX>(a, pc, hs, N, m, xrow, X, acc);
+        sb_spmm_vec_piece<WG, RTMAX>(a, pc, hs, N, m, xrow, X, acc, spv, spx);
     }
     if (m.active) {
 #pragma unroll
@@ -469,7 +529,9 @@ template <int WG, int RTMAX, int NRHS, int XCAP>
 __device__ __forceinline__ void sb_spmm_f32_body(const SbSpmmArgs &a)
 {
     __shared__ int xrow[XCAP + 4];
+    __shared__ float spv[SB_SPMM_SPCHUNK];
+    __shared__ int spx[SB_SPMM_SPCHUNK];
     __shared__ int next_tile;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow); });
+                 [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow, spv, spx); });
 }

@@ -70,21 +70,39 @@ __device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
     for (int s = 0; s < SPLIT; ++s) t[s] = sb_as_bf16x8(sb_u4{w[s][0], w[s][1], w[s][2], w[s][3]});
 }
 
-// Stages k-step ks of an MFMA piece: B[s][n * SB_KP + k], k < 32, s < SPLIT.
-template <int WG, int SPLIT>
-__device__ __forceinline__ void sb_stage_b(const float *__restrict__ X, int N, const int *xrow, int ncols,
-                                           int ks, uint16_t *B, int term_stride)
+// X staging of an MFMA piece, split in two phases so the global loads of
+// k-step s+1 are in flight while k-step s computes: sb_fetch_b loads the
+// thread's X values of a k-step into registers, sb_store_b splits them and
+// writes B[s][n * SB_KP + k] (k < 32, s < SPLIT) to LDS.
+template <int WG, int UPT>
+__device__ __forceinline__ void sb_fetch_b(const float *__restrict__ X, int N, const int *xrow, int ncols, int ks,
+                                           float (*f)[8])
 {
     const int units = N * 4;
-    for (int u = threadIdx.x; u < units; u += WG) {
+#pragma unroll
+    for (int uu = 0; uu < UPT; ++uu) {
+        const int u = threadIdx.x + uu * WG;
         const int kg = u / N;
         const int n = u - kg * N;
         const int k0 = ks * 32 + kg * 8;
-        float f[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) f[j] = (k0 + j < ncols) ? X[(size_t)xrow[k0 + j] * N + n] : 0.f;
+        for (int j = 0; j < 8; ++j)
+            f[uu][j] = (u < units && k0 + j < ncols) ? X[(size_t)xrow[k0 + j] * N + n] : 0.f;
+    }
+}
+
+template <int WG, int UPT, int SPLIT>
+__device__ __forceinline__ void sb_store_b(int N, float (*f)[8], uint16_t *B, int term_stride)
+{
+    const int units = N * 4;
+#pragma unroll
+    for (int uu = 0; uu < UPT; ++uu) {
+        const int u = threadIdx.x + uu * WG;
+        if (u >= units) continue;
+        const int kg = u / N;
+        const int n = u - kg * N;
         sb_bf16x8 t[SPLIT];
-        sb_split8<SPLIT>(f, t);
+        sb_split8<SPLIT>(f[uu], t);
 #pragma unroll
         for (int s = 0; s < SPLIT; ++s)
             *reinterpret_cast<sb_bf16x8 *>(B + s * term_stride + n * SB_KP + kg * 8) = t[s];
@@ -114,9 +132,11 @@ struct SbMfmaCfg {
     static constexpr int RT0 = (SB_MFMA_MAXROWS * (NMAX / 4) + WG - 1) / WG;
     static constexpr int RTMAX = RT0 < 1 ? 1 : RT0;
     static constexpr int TERM_STRIDE = NMAX * SB_KP;            // bf16 elements per staged term
+    static constexpr int UPT = (NMAX * 4 + WG - 1) / WG;         // X staging units per thread
     static constexpr int BSTAGE_BYTES = SPLIT * TERM_STRIDE * 2;
     static constexpr int YTILE_BYTES = SB_MFMA_MAXROWS * NMAX * 4;
-    static constexpr int SMEM_BYTES = BSTAGE_BYTES > YTILE_BYTES ? BSTAGE_BYTES : YTILE_BYTES;
+    static constexpr int SMEM0 = BSTAGE_BYTES > YTILE_BYTES ? BSTAGE_BYTES : YTILE_BYTES;
+    static constexpr int SMEM_BYTES = SMEM0 > 8 * SB_SPMM_SPCHUNK ? SMEM0 : 8 * SB_SPMM_SPCHUNK;
 };
 
 template <int WG, int NRHS, int SPLIT>
@@ -161,15 +181,29 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
         if (pc.kind == SB_PIECE_MFMA) {
             has_mfma = true;
             const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
+            float xf[Cfg::UPT][8];
+            sb_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+            sb_f4 n0 = a0, n1 = a1;
+            sb_fetch_b<WG, Cfg::UPT>(X, N, xrow, pc.ncols, 0, xf);
+            if (mactive) {
+                const int off = (mi * pc.ksteps * 64 + lane) * 2;
+                n0 = __builtin_nontemporal_load(&A[off]);
+                n1 = __builtin_nontemporal_load(&A[off + 1]);
+            }
             for (int ks = 0; ks < pc.ksteps; ++ks) {
-                sb_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
-                if (mactive) {
-                    const int off = ((mi * pc.ksteps + ks) * 64 + lane) * 2;
-                    a0 = __builtin_nontemporal_load(&A[off]);
-                    a1 = __builtin_nontemporal_load(&A[off + 1]);
-                }
-                sb_stage_b<WG, SPLIT>(X, N, xrow, pc.ncols, ks, B, Cfg::TERM_STRIDE);
+                __syncthreads();                       // B free: previous k-step consumed
+                sb_store_b<WG, Cfg::UPT, SPLIT>(N, xf, B, Cfg::TERM_STRIDE);
+                a0 = n0;
+                a1 = n1;
                 __syncthreads();
+                if (ks + 1 < pc.ksteps) {              // next k-step's loads overlap this one's MFMAs
+                    sb_fetch_b<WG, Cfg::UPT>(X, N, xrow, pc.ncols, ks + 1, xf);
+                    if (mactive) {
+                        const int off = ((mi * pc.ksteps + ks + 1) * 64 + lane) * 2;
+                        n0 = __builtin_nontemporal_load(&A[off]);
+                        n1 = __builtin_nontemporal_load(&A[off + 1]);
+                    }
+                }
                 if (mactive) {
                     const float af[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
                     sb_bf16x8 at[SPLIT];
@@ -187,11 +221,12 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
                         }
                     }
                 }
-                __syncthreads();
             }
         } else {
             has_vec = true;
-            if (vm.active) sb_spmm_vec_piece<Cfg::RTMAX>(a, pc, hs, N, vm, xrow, X, vacc);
+            // LDS scratch for the sparse entries: the B staging area is free here
+            sb_spmm_vec_piece<WG, Cfg::RTMAX>(a, pc, hs, N, vm, xrow, X, vacc, reinterpret_cast<float *>(smem),
+                                              reinterpret_cast<int *>(smem) + SB_SPMM_SPCHUNK);
         }
     }
 
