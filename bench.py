@@ -153,6 +153,16 @@ def bench_spmm(args, runtime, info):
     print(json.dumps(line), flush=True)
 
 
+def spmv_options(args):
+    """Plan options: the autotuned table's winners unless given on the command line."""
+    from sable_amd.autotune import tuned_options
+    opts = {k: v for k, v in tuned_options("spmv").items() if k in ("mode", "tile_bytes", "unroll", "wg")}
+    for k in ("mode", "tile_bytes", "unroll"):
+        if getattr(args, k) is not None:
+            opts[k] = getattr(args, k)
+    return opts
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,8 +172,9 @@ def main():
     ap.add_argument("--warm", action="store_true", help="no cache flush between launches")
     ap.add_argument("--routing", default="plain", choices=["plain", "learned", "both"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", type=int, default=2)
-    ap.add_argument("--tile-bytes", type=int, default=0)
+    ap.add_argument("--mode", type=int, default=None, help="default: the autotuned value")
+    ap.add_argument("--tile-bytes", type=int, default=None, help="default: the autotuned value")
+    ap.add_argument("--unroll", type=int, default=None, help="default: the autotuned value")
     ap.add_argument("--threshold", type=float, default=15)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -177,6 +188,7 @@ def main():
 
     runtime.init(0)
     info = runtime.device_info()
+    info.update(runtime.bandwidth_probe(256 << 20, reps=3))
     log(f"device: {info}")
     t0 = time.time()
     if args.workload == "c2":
@@ -190,8 +202,9 @@ def main():
         return bench_spmm(args, runtime, info)
     log(f"workload generated in {time.time() - t0:.1f}s: {len(vbr.val)} stored values, {len(vbr.bindx)} blocks")
     routes = threshold_routes(vbr.val, vbr.indx, args.threshold)
+    opts = spmv_options(args)
     t1 = time.time()
-    plan = runtime.Plan.from_vbr(vbr, routes, mode=args.mode, tile_bytes=args.tile_bytes)
+    plan = runtime.Plan.from_vbr(vbr, routes, **opts)
     inspector_s = time.time() - t1
     st = plan.stats
     log(f"plan: {st['ntiles']} tiles, rtc={st['rtc_used']}, grid={st['grid']}, bytes_alg={st['bytes_alg']}, "
@@ -246,8 +259,9 @@ def main():
         "warm_ms": float(np.mean(warm)),
         "warm_gbs": st["bytes_alg"] / (float(np.mean(warm)) * 1e-3) / 1e9,
         "error_vs_fp64": errs, "bytes_alg": st["bytes_alg"], "inspector_s": inspector_s,
-        "plan": {k: st[k] for k in ("ntiles", "npieces", "dense_values", "sparse_nnz", "rowptr_entries",
-                                    "nblocks_dense", "nblocks_sparse", "rtc_used", "grid", "mode")},
+        "plan": dict({k: st[k] for k in ("ntiles", "npieces", "dense_values", "sparse_nnz", "rowptr_entries",
+                                         "nblocks_dense", "nblocks_sparse", "rtc_used", "grid", "mode", "wg")},
+                     options=opts),
         "device": info,
     }
     print(json.dumps(line), flush=True)

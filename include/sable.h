@@ -76,7 +76,6 @@ typedef struct {
     int64_t tile_bytes;       /* target algorithmic bytes per tile */
     int32_t max_seg_rows;     /* upper bound on tile height (<= wg) */
     int32_t min_seg_rows;     /* lower bound on tile height when splitting */
-    int32_t sp_chunk;         /* SpMV sparse entries per LDS product pass (<= 1024; 0 = 1024) */
     int32_t mode;             /* SABLE_MODE_*; -1 = default (queue) */
     int32_t grid;             /* persistent grid size; 0 = 8 per CU */
     int32_t unroll;           /* SpMV dense stream unroll (1,2,4,8); 0 = 4 */
@@ -97,7 +96,9 @@ typedef struct {
     int64_t mfma_values;      /* sum over MFMA-routed blocks of r*c */
     int64_t mfma_padded;      /* MFMA-ordered FP32 elements, padding included */
     int64_t sparse_nnz;       /* nonzeros stored in sparse pieces */
-    int64_t rowptr_entries;   /* sparse row pointer entries */
+    int64_t sparse_padded;    /* sparse slots as laid out (SpMV wave slices pad rows) */
+    int64_t rowptr_entries;   /* sparse index entries: CSR row pointers (SpMM) or
+                                 row orders and wavefront tables (SpMV) */
     int64_t descriptor_bytes; /* tiles + pieces + ranges */
     int64_t bytes_alg;        /* compulsory bytes of the executed format */
     int64_t flops_alg;        /* 2 * nnz * N */
@@ -185,6 +186,11 @@ int sable_adam_step(float *params, const float *grad, float *m, float *v, int64_
  * kind: 0 SpMV, 1 SpMM FP32, 2 SpMM MFMA (split 3), 3 SpMM MFMA (split 2). */
 int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int rtmax,
                          const char *arch);
+
+/* Achievable HBM rates (no reference counterpart): best of `reps` launches
+ * of a streaming read and a streaming copy over `bytes` (GB/s; the copy
+ * counts read + write bytes). */
+int sable_bandwidth_probe(int64_t bytes, int reps, float *read_gbs, float *copy_gbs);
 
 /* Device memory helpers for callers without another allocator. */
 int sable_malloc(void **ptr, int64_t bytes);

@@ -47,7 +47,7 @@ class SableVbr(C.Structure):
 
 class SablePlanOpts(C.Structure):
     _fields_ = [("op", I32), ("n_rhs", I32), ("wg", I32), ("xcap", I32), ("tile_bytes", I64),
-                ("max_seg_rows", I32), ("min_seg_rows", I32), ("sp_chunk", I32),
+                ("max_seg_rows", I32), ("min_seg_rows", I32),
                 ("mode", I32), ("grid", I32), ("unroll", I32), ("use_rtc", I32),
                 ("host_only", I32), ("mfma_split", I32)]
 
@@ -57,7 +57,8 @@ class SablePlanStats(C.Structure):
                 ("nxranges", I32), ("n_rhs", I32), ("nblocks", I64), ("nblocks_dense", I64),
                 ("nblocks_sparse", I64), ("nblocks_mfma", I64), ("nnz", I64),
                 ("dense_values", I64), ("dense_padded", I64), ("mfma_values", I64),
-                ("mfma_padded", I64), ("sparse_nnz", I64), ("rowptr_entries", I64),
+                ("mfma_padded", I64), ("sparse_nnz", I64), ("sparse_padded", I64),
+                ("rowptr_entries", I64),
                 ("descriptor_bytes", I64), ("bytes_alg", I64), ("flops_alg", I64),
                 ("device_bytes", I64), ("rtc_used", I32), ("mode", I32), ("grid", I32),
                 ("wg", I32)]
@@ -103,6 +104,7 @@ _SIGS = {
     "sable_mlp_backward": (I32, [P, I32, I32, I32, P, I32, P, P, P]),
     "sable_mlp_mse_grad": (I32, [P, P, P, I64, C.c_float, P]),
     "sable_adam_step": (I32, [P, P, P, P, I64, C.c_float, C.c_float, C.c_float, C.c_float, I32]),
+    "sable_bandwidth_probe": (I32, [I64, I32, F32P, F32P]),
     "sable_malloc": (I32, [C.POINTER(P), I64]),
     "sable_free": (I32, [P]),
     "sable_memcpy_h2d": (I32, [P, P, I64]),

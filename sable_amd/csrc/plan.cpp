@@ -37,7 +37,6 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 48 * 1024 : 64 * 1024;
     if (o.max_seg_rows <= 0) o.max_seg_rows = o.wg;
     if (o.min_seg_rows <= 0) o.min_seg_rows = (o.op == SABLE_OP_SPMV) ? 8 : 16;
-    if (o.sp_chunk <= 0 || o.sp_chunk > 1024) o.sp_chunk = 1024;
     if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_QUEUE;
     if (o.unroll <= 0) o.unroll = 4;
     if (o.mfma_split != 2) o.mfma_split = 3;
@@ -138,6 +137,153 @@ void dense_stream_params(int hs, int wg, int32_t &tpar, int32_t &mcols)
 }
 
 }  // namespace
+
+// Nonzeros of rows [s0, s0+hs) of a block row (height h) restricted to one
+// column group, row by row in column order (local column = position in the
+// group's concatenated columns).  Returns false when there are none.
+struct SparseRows {
+    int32_t group = 0;
+    std::vector<int32_t> rp;      // hs + 1 row pointers
+    std::vector<float> val;
+    std::vector<uint16_t> col;
+};
+
+bool collect_sparse_rows(const SableVbr &v, const std::vector<ColSeg> &g, int32_t h, int32_t s0, int32_t hs,
+                         SparseRows &out)
+{
+    std::vector<int32_t> cnt(hs, 0);
+    for (auto &cs : g) {
+        const float *bv = v.val + v.indx[cs.k];
+        for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c) {
+            const float *col = bv + (int64_t)c * h + s0;
+            for (int32_t r = 0; r < hs; ++r) cnt[r] += (col[r] != 0.0f);
+        }
+    }
+    out.rp.assign(hs + 1, 0);
+    for (int32_t r = 0; r < hs; ++r) out.rp[r + 1] = out.rp[r] + cnt[r];
+    const int32_t tot = out.rp[hs];
+    if (tot == 0) return false;
+    out.val.resize(tot);
+    out.col.resize(tot);
+    std::vector<int32_t> cur(out.rp.begin(), out.rp.end() - 1);
+    int32_t lc = 0;
+    for (auto &cs : g) {
+        const float *bv = v.val + v.indx[cs.k];
+        for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c, ++lc) {
+            const float *col = bv + (int64_t)c * h + s0;
+            for (int32_t r = 0; r < hs; ++r) {
+                if (col[r] != 0.0f) {
+                    out.val[cur[r]] = col[r];
+                    out.col[cur[r]] = (uint16_t)lc;
+                    ++cur[r];
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// SpMV sparse layout ("wave slices").  The tile's rows are ordered by total
+// sparse length (longest first, ties by row) and dealt in runs of C = 64/G
+// rows to the wg/64 wavefronts; each row gets G lanes.  A wavefront step is
+// 256 consecutive entries (64 lanes x 4): lane q*G + g of the wavefront
+// takes 4-entry chunk c = j*G + g of its row at step j, so every step is one
+// coalesced 1 KiB value load and 512 B index load per wavefront.  Each
+// wavefront's steps are contiguous; a row is padded to the wavefront's step
+// count with value 0 at the zero pad slot of the staged x slice (local
+// column ncols).  Per piece the table holds {step offset, steps} for each
+// wavefront.  G is chosen per tile to minimise padded slots plus half the
+// slots the longest wavefront keeps the tile busy for.  Returns the cost.
+double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::vector<SparseRows> &sp,
+                        const std::vector<std::vector<ColSeg>> &sgroups, int32_t hs, int32_t wg)
+{
+    const int nw = wg / 64;
+    std::vector<int64_t> tot(hs, 0);
+    for (auto &sr : sp)
+        for (int32_t r = 0; r < hs; ++r) tot[r] += sr.rp[r + 1] - sr.rp[r];
+    std::vector<int32_t> order(hs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return tot[x] > tot[y]; });
+
+    // smallest power-of-two run that covers the rows with nw wavefronts
+    int cmin = 1;
+    while (cmin * nw < hs) cmin <<= 1;
+    auto widths = [&](int glog, std::vector<std::vector<int32_t>> *out) {
+        const int G = 1 << glog, C = 64 >> glog;
+        int64_t padded = 0, crit = 0;
+        std::vector<int64_t> wsteps(nw, 0);
+        for (size_t p = 0; p < sp.size(); ++p) {
+            std::vector<int32_t> w(nw, 0);
+            for (int wv = 0; wv < nw; ++wv) {
+                int32_t m = 0;
+                for (int q = 0; q < C && wv * C + q < hs; ++q) {
+                    const int32_t r = order[wv * C + q];
+                    const int32_t len = sp[p].rp[r + 1] - sp[p].rp[r];
+                    m = std::max(m, (len + 4 * G - 1) / (4 * G));
+                }
+                w[wv] = m;
+                wsteps[wv] += m;
+                padded += 256LL * m;
+            }
+            if (out) out->push_back(w);
+        }
+        for (int wv = 0; wv < nw; ++wv) crit = std::max(crit, wsteps[wv]);
+        return std::make_pair(padded, crit);
+    };
+    int best = 0;
+    double best_cost = 0;
+    for (int glog = 0; (64 >> glog) >= cmin; ++glog) {
+        const auto pc = widths(glog, nullptr);
+        const double c = (double)pc.first + 0.5 * 256.0 * nw * (double)pc.second;
+        if (glog == 0 || c < best_cost) { best = glog; best_cost = c; }
+    }
+    std::vector<std::vector<int32_t>> W;
+    widths(best, &W);
+    const int G = 1 << best, C = 64 >> best;
+    t.sp_glog = best;
+    t.sp_map = (int32_t)P.srowptr.size();
+    P.srowptr.insert(P.srowptr.end(), order.begin(), order.end());
+    S.rowptr_entries += hs;
+    double cost = 4.0 * hs;
+    for (size_t p = 0; p < sp.size(); ++p) {
+        const SparseRows &sr = sp[p];
+        SbPiece pc{};
+        pc.kind = SB_PIECE_SPARSE;
+        pc.ncols = group_width(sgroups[sr.group]);
+        pc.data_off = (int64_t)P.sval.size();
+        pc.rowptr_off = (int32_t)P.srowptr.size();
+        int64_t steps = 0;
+        for (int wv = 0; wv < nw; ++wv) {
+            P.srowptr.push_back((int32_t)steps);
+            P.srowptr.push_back(W[p][wv]);
+            steps += W[p][wv];
+        }
+        S.rowptr_entries += 2 * nw;
+        P.sval.resize(P.sval.size() + 256 * steps, 0.0f);
+        P.sidx.resize(P.sidx.size() + 256 * steps, (uint16_t)pc.ncols);
+        float *dv = P.sval.data() + pc.data_off;
+        uint16_t *di = P.sidx.data() + pc.data_off;
+        int64_t woff = 0;
+        for (int wv = 0; wv < nw; ++wv) {
+            for (int q = 0; q < C && wv * C + q < hs; ++q) {
+                const int32_t r = order[wv * C + q];
+                for (int32_t e = sr.rp[r]; e < sr.rp[r + 1]; ++e) {
+                    const int32_t i = e - sr.rp[r];          // position in the row
+                    const int32_t c = i / 4, k = i % 4;      // chunk, slot
+                    const int64_t pos = (woff + c / G) * 256 + (int64_t)(q * G + c % G) * 4 + k;
+                    dv[pos] = sr.val[e];
+                    di[pos] = sr.col[e];
+                }
+            }
+            woff += W[p][wv];
+        }
+        S.sparse_nnz += (int64_t)sr.val.size();
+        push_xranges(P, sgroups[sr.group], pc);
+        P.pieces.push_back(pc);
+        cost += 6.0 * 256.0 * (double)steps + 8.0 * nw;
+    }
+    return cost;
+}
 
 int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o,
                     HostPlan &P, std::string &err)
@@ -292,61 +438,40 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                 P.pieces.push_back(pc);
                 cost += 4.0 * hs * pc.ncols;
             }
-            // sparse pieces
-            int64_t seg_sp = 0;
-            for (auto &g : sgroups) {
-                // count
-                std::vector<int32_t> cnt(hs, 0);
-                for (auto &cs : g) {
-                    const float *bv = v.val + v.indx[cs.k];
-                    for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c) {
-                        const float *col = bv + (int64_t)c * h + s0;
-                        for (int32_t r = 0; r < hs; ++r) cnt[r] += (col[r] != 0.0f);
+            // sparse pieces: the rows' nonzeros in column order, per group
+            std::vector<SparseRows> sp;
+            for (size_t gi = 0; gi < sgroups.size(); ++gi) {
+                SparseRows sr;
+                if (collect_sparse_rows(v, sgroups[gi], h, s0, hs, sr)) {
+                    sr.group = (int32_t)gi;
+                    sp.push_back(std::move(sr));
+                }
+            }
+            t.sp_glog = 0;
+            t.sp_map = -1;
+            if (!sp.empty()) {
+                if (spmm) {
+                    for (auto &sr : sp) {
+                        SbPiece pc{};
+                        pc.kind = SB_PIECE_SPARSE;
+                        pc.ncols = group_width(sgroups[sr.group]);
+                        pc.data_off = (int64_t)P.sval.size();
+                        pc.rowptr_off = (int32_t)P.srowptr.size();
+                        P.srowptr.insert(P.srowptr.end(), sr.rp.begin(), sr.rp.end());
+                        P.sval.insert(P.sval.end(), sr.val.begin(), sr.val.end());
+                        P.sidx.insert(P.sidx.end(), sr.col.begin(), sr.col.end());
+                        const int64_t tot = (int64_t)sr.val.size();
+                        S.sparse_nnz += tot;
+                        S.rowptr_entries += hs + 1;
+                        push_xranges(P, sgroups[sr.group], pc);
+                        P.pieces.push_back(pc);
+                        cost += 6.0 * tot + 4.0 * (hs + 1);
                     }
+                } else {
+                    cost += emit_wave_slices(P, S, t, sp, sgroups, hs, o.wg);
                 }
-                int64_t tot = 0;
-                for (int32_t r = 0; r < hs; ++r) tot += cnt[r];
-                if (tot == 0) continue;   // no nonzeros in these rows: contributes nothing
-                SbPiece pc{};
-                pc.kind = SB_PIECE_SPARSE;
-                pc.ncols = group_width(g);
-                pc.data_off = (int64_t)P.sval.size();
-                pc.rowptr_off = (int32_t)P.srowptr.size();
-                std::vector<int32_t> cur(hs);
-                int32_t acc = 0;
-                for (int32_t r = 0; r < hs; ++r) {
-                    P.srowptr.push_back(acc);
-                    cur[r] = acc;
-                    acc += cnt[r];
-                }
-                P.srowptr.push_back(acc);
-                P.sval.resize(P.sval.size() + tot);
-                P.sidx.resize(P.sidx.size() + tot);
-                float *sv = P.sval.data() + pc.data_off;
-                uint16_t *si = P.sidx.data() + pc.data_off;
-                int32_t lc = 0;
-                for (auto &cs : g) {
-                    const float *bv = v.val + v.indx[cs.k];
-                    for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c, ++lc) {
-                        const float *col = bv + (int64_t)c * h + s0;
-                        for (int32_t r = 0; r < hs; ++r) {
-                            if (col[r] != 0.0f) {
-                                sv[cur[r]] = col[r];
-                                si[cur[r]] = (uint16_t)lc;
-                                ++cur[r];
-                            }
-                        }
-                    }
-                }
-                S.sparse_nnz += tot;
-                S.rowptr_entries += hs + 1;
-                seg_sp += tot;
-                push_xranges(P, g, pc);
-                P.pieces.push_back(pc);
-                cost += 6.0 * tot + 4.0 * (hs + 1);
             }
             t.piece_end = (int32_t)P.pieces.size();
-            t.sp_chunk = o.sp_chunk;
             if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
             P.tiles.push_back(t);
             P.tile_cost.push_back(cost);
@@ -370,6 +495,7 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
     S.npieces = (int32_t)P.pieces.size();
     S.nxranges = (int32_t)P.xr.size();
     S.dense_padded = (int64_t)P.dval.size() - S.mfma_padded;
+    S.sparse_padded = (int64_t)P.sval.size();
     S.descriptor_bytes = (int64_t)(P.tiles.size() * sizeof(SbTile) + P.pieces.size() * sizeof(SbPiece) +
                                    P.xr.size() * sizeof(SbXRange));
     S.bytes_alg = 4 * S.dense_values + 4 * S.mfma_values + 6 * S.sparse_nnz + 4 * S.rowptr_entries +

@@ -31,7 +31,7 @@
 // ---------------------------------------------------------------------------
 // ahead-of-time generic instances (fallback when hipRTC is unavailable)
 
-extern "C" __global__ void __launch_bounds__(256) sb_spmv_aot(SbSpmvArgs a)
+extern "C" __global__ void __launch_bounds__(256, SB_SPMV_WAVES(4)) sb_spmv_aot(SbSpmvArgs a)
 {
     sb_spmv_body<256, 4, 4096>(a);
 }
@@ -56,6 +56,35 @@ extern "C" __global__ void sb_fill_u32(uint32_t *p, int64_t n, uint32_t v)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// Bandwidth probes: a streaming read (float4 loads, one partial sum stored
+// per thread so nothing is optimised away) and a streaming copy.  They give
+// the achievable HBM rates of the device and the bytes a known-size copy
+// shows in the traffic counters.
+extern "C" __global__ void __launch_bounds__(256) sb_read_probe(const sb_f4 *__restrict__ p, int64_t n4,
+                                                                float *__restrict__ out)
+{
+    sb_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const sb_f4 a = __builtin_nontemporal_load(&p[i]);
+        const sb_f4 b = __builtin_nontemporal_load(&p[i + stride]);
+        const sb_f4 c = __builtin_nontemporal_load(&p[i + 2 * stride]);
+        const sb_f4 d = __builtin_nontemporal_load(&p[i + 3 * stride]);
+        acc += a + b + c + d;
+    }
+    for (; i < n4; i += stride) acc += __builtin_nontemporal_load(&p[i]);
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
+}
+
+extern "C" __global__ void __launch_bounds__(256) sb_copy_probe(const sb_f4 *__restrict__ src,
+                                                                sb_f4 *__restrict__ dst, int64_t n4)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
 }
 
 // embedded kernel sources for hipRTC (generated at build time)
@@ -228,7 +257,7 @@ std::string spec_defines(const Spec &s)
 }
 
 const char *kStubSpmv =
-    "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmv_rtc(SbSpmvArgs a)\n"
+    "extern \"C\" __global__ void __launch_bounds__(SB_WG, SB_SPMV_WAVES(SB_UNROLL)) sb_spmv_rtc(SbSpmvArgs a)\n"
     "{ sb_spmv_body<SB_WG, SB_UNROLL, SB_XCAP>(a); }\n";
 const char *kStubSpmmF32 =
     "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_f32_rtc(SbSpmmArgs a)\n"
@@ -649,8 +678,8 @@ int sable_plan_export(const SablePlan *p, int which, void *dst, int64_t *bytes)
     case 1: hsrc = h.pieces.data(); n = h.pieces.size() * sizeof(SbPiece); break;
     case 2: hsrc = h.xr.data(); n = h.xr.size() * sizeof(SbXRange); break;
     case 3: hsrc = h.dval.data(); dsrc = p->d_dval; n = (p->stats.dense_padded + p->stats.mfma_padded) * 4; break;
-    case 4: hsrc = h.sval.data(); dsrc = p->d_sval; n = p->stats.sparse_nnz * 4; break;
-    case 5: hsrc = h.sidx.data(); dsrc = p->d_sidx; n = p->stats.sparse_nnz * 2; break;
+    case 4: hsrc = h.sval.data(); dsrc = p->d_sval; n = p->stats.sparse_padded * 4; break;
+    case 5: hsrc = h.sidx.data(); dsrc = p->d_sidx; n = p->stats.sparse_padded * 2; break;
     case 6: hsrc = h.srowptr.data(); dsrc = p->d_srowptr; n = p->stats.rowptr_entries * 4; break;
     default: return fail(SABLE_ERR_INVALID, "unknown export id");
     }
@@ -796,7 +825,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
                 pc.data_off += dval_n;
                 mfma = mfma || pc.kind == SB_PIECE_MFMA;
             } else {
-                pc.data_off += sval_n;
+                pc.data_off += sval_n;   // multiples of 256 entries keep SpMV float4 alignment
                 pc.rowptr_off += (int32_t)srow_n;
             }
             maxcols = std::max(maxcols, pc.ncols);
@@ -808,11 +837,12 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
             tt.piece_begin += pbase;
             tt.piece_end += pbase;
             tt.mat = i;
+            if (tt.sp_map >= 0) tt.sp_map += (int32_t)srow_n;
             tiles.push_back(tt);
             cost.push_back(p->host.tile_cost[t]);
         }
         dval_n += p->stats.dense_padded + p->stats.mfma_padded;
-        sval_n += p->stats.sparse_nnz;
+        sval_n += p->stats.sparse_padded;
         srow_n += p->stats.rowptr_entries;
     }
     // global LPT order
@@ -843,8 +873,8 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     using P2 = std::pair<const void *, int64_t>;
     CHECKRC(cat(&g->d_dval, dval_n,
                 [](SablePlan *p) { return P2(p->d_dval, p->stats.dense_padded + p->stats.mfma_padded); }, 4));
-    CHECKRC(cat(&g->d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_nnz); }, 4));
-    CHECKRC(cat(&g->d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_nnz); }, 2));
+    CHECKRC(cat(&g->d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_padded); }, 4));
+    CHECKRC(cat(&g->d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_padded); }, 2));
     CHECKRC(cat(&g->d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
     HIPOK(hipMalloc((void **)&g->d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(g->d_queue, 0, 2 * sizeof(int32_t)));
@@ -992,6 +1022,49 @@ int sable_adam_step(float *params, const float *grad, float *m, float *v, int64_
                        n, lr, beta1, beta2, eps, step);
     HIPOK(hipGetLastError());
     HIPOK(hipDeviceSynchronize());
+    return SABLE_OK;
+}
+
+// ---- bandwidth probe
+
+int sable_bandwidth_probe(int64_t bytes, int reps, float *read_gbs, float *copy_gbs)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (bytes < (1 << 20) || reps <= 0) return fail(SABLE_ERR_INVALID, "bytes >= 1 MiB and reps > 0 needed");
+    const int64_t n4 = bytes / 16;
+    sb_f4 *a = nullptr, *b = nullptr;
+    float *out = nullptr;
+    const int grid = std::max(1, g_ncu) * 8;
+    HIPOK(hipMalloc((void **)&a, n4 * 16));
+    HIPOK(hipMalloc((void **)&b, n4 * 16));
+    HIPOK(hipMalloc((void **)&out, (size_t)grid * 256 * 4));
+    HIPOK(hipMemset(a, 0, n4 * 16));
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    float best_r = 0.f, best_c = 0.f;
+    for (int r = 0; r < reps + 1; ++r) {
+        float ms = 0.f;
+        hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL(sb_read_probe, dim3(grid), dim3(256), 0, nullptr, a, n4, out);
+        hipEventRecord(e1, nullptr);
+        HIPOK(hipEventSynchronize(e1));
+        hipEventElapsedTime(&ms, e0, e1);
+        if (r) best_r = std::max(best_r, (float)(n4 * 16 / (ms * 1e-3) / 1e9));
+        hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL(sb_copy_probe, dim3(grid), dim3(256), 0, nullptr, a, b, n4);
+        hipEventRecord(e1, nullptr);
+        HIPOK(hipEventSynchronize(e1));
+        hipEventElapsedTime(&ms, e0, e1);
+        if (r) best_c = std::max(best_c, (float)(2 * n4 * 16 / (ms * 1e-3) / 1e9));
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    hipFree(out);
+    if (read_gbs) *read_gbs = best_r;
+    if (copy_gbs) *copy_gbs = best_c;
     return SABLE_OK;
 }
 

@@ -28,7 +28,9 @@ struct SbTile {
     int32_t mat;             // matrix slot (batched plans); 0 otherwise
     int32_t dense_tpar;      // T': active lanes of the flat dense stream
     int32_t dense_mcols;     // M : columns advanced per stream step (4T' = M*nrows)
-    int32_t sp_chunk;        // sparse entries per product pass (<= the LDS chunk)
+    int32_t sp_glog;         // SpMV sparse: log2 of the lanes per row (G)
+    int32_t sp_map;          // SpMV sparse: srowptr offset of the row order
+                             // (nrows local rows, longest sparse row first); -1 if none
 };
 
 // A panel piece of a tile: the tile's rows restricted to a group of blocks
@@ -41,7 +43,9 @@ struct SbPiece {
     int32_t ncols;           // staged columns (x slice length)
     int32_t xr_begin;        // staging ranges xr[xr_begin, xr_end)
     int32_t xr_end;
-    int32_t rowptr_off;      // sparse: srowptr[rowptr_off .. +nrows] (relative to data_off)
+    int32_t rowptr_off;      // sparse, SpMM: CSR srowptr[rowptr_off .. +nrows] (relative
+                             // to data_off); SpMV: per-wavefront {step offset,
+                             // steps} pairs at srowptr[rowptr_off ..]
     int32_t ksteps;          // mfma: number of 32-wide k steps (ncols padded)
 };
 

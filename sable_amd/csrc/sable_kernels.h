@@ -29,6 +29,7 @@
 
 typedef float sb_f4 __attribute__((ext_vector_type(4)));
 typedef unsigned int sb_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned int sb_u2 __attribute__((ext_vector_type(2)));
 
 #define SB_MODE_GRID 0      // one workgroup per tile
 #define SB_MODE_STATIC 1    // persistent, tile = wg + k * grid
@@ -133,17 +134,26 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
 // pipelined (two register batches of UNROLL float4), and the first batch of
 // a group's first dense piece is issued before the staging barrier.
 //
-// Sparse piece = CSR over the tile's rows with uint16 column indices into the
-// staged x slice.  Entries are read in one flat coalesced pass per chunk of
-// SB_SPCHUNK entries, their products v*x[col] written to LDS, and each row's
-// owner thread then sums its CSR range in column order.
+// Sparse piece = wave slices (laid out by the host plan): the tile's rows,
+// longest first, are dealt in runs of C = 64/G rows to the wavefronts, G
+// lanes per row; at step j lane q*G+g holds entries 4(jG+g) .. 4(jG+g)+3 of
+// its row (uint16 columns into the staged x slice).  A wavefront streams its
+// steps as one coalesced float4 + 4 x uint16 load per lane, accumulates in a
+// register, folds the G lanes of a row with a fixed xor butterfly and adds
+// the row sum to the row's LDS slot spy[r].  The same lane owns a row in
+// every sparse piece of the tile, so spy needs no barrier between pieces.
 //
 // Row r's result is (dense partial sums of the M column groups, in group
-// order) + (sparse products in CSR order): deterministic, no atomics.
+// order) + (sparse sums, piece by piece): deterministic, no atomics.
 
-#ifndef SB_SPCHUNK
-#define SB_SPCHUNK 1024
+// Sparse steps per pipelined batch, and the waves per SIMD the SpMV kernel
+// is compiled for (its VGPR budget) given the dense unroll: two sparse steps
+// keep the kernel within 80 VGPRs at unroll 2 (6 waves) and 96 at unroll 4
+// (5 waves) without spills.
+#ifndef SB_SPUNROLL
+#define SB_SPUNROLL 2
 #endif
+#define SB_SPMV_WAVES(U) ((U) <= 2 ? 6 : (U) <= 4 ? 5 : 1)
 
 struct SbStreamLane {
     int c0, c1, c2, c3;    // column offsets of this lane's four elements
@@ -244,8 +254,64 @@ __device__ __forceinline__ void sb_stage_group(const SbPiece *__restrict__ piece
     }
 }
 
+// One wavefront's share of a sparse piece.  Loads are unconditional (the step
+// index is clamped to the last step) and software pipelined one batch ahead;
+// steps past the wavefront's count are masked to value 0 at the slice's zero
+// pad slot xs[pad].
+template <int SU>
+__device__ __forceinline__ void sb_sparse_piece(const float *__restrict__ sval, const uint16_t *__restrict__ sidx,
+                                                const int32_t *__restrict__ tab, const int32_t *__restrict__ map,
+                                                int hs, int glog, int pad, const float *xs, float *spy)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int off = tab[2 * wave];
+    const int nst = tab[2 * wave + 1];
+    if (nst <= 0) return;
+    const sb_f4 *__restrict__ V = reinterpret_cast<const sb_f4 *>(sval) + (int64_t)off * 64 + lane;
+    const sb_u2 *__restrict__ I = reinterpret_cast<const sb_u2 *>(sidx) + (int64_t)off * 64 + lane;
+    const unsigned padpair = (unsigned)pad | ((unsigned)pad << 16);
+    sb_f4 cv[SU];
+    sb_u2 ci[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+        const int j = min(u, nst - 1);
+        cv[u] = __builtin_nontemporal_load(V + (int64_t)j * 64);
+        ci[u] = __builtin_nontemporal_load(I + (int64_t)j * 64);
+    }
+    float acc = 0.f;
+    for (int j0 = 0; j0 < nst; j0 += SU) {
+        sb_f4 nv[SU];
+        sb_u2 ni[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int j = min(j0 + SU + u, nst - 1);
+            nv[u] = __builtin_nontemporal_load(V + (int64_t)j * 64);
+            ni[u] = __builtin_nontemporal_load(I + (int64_t)j * 64);
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const bool ok = j0 + u < nst;
+            const unsigned lo = ok ? ci[u].x : padpair;
+            const unsigned hi = ok ? ci[u].y : padpair;
+            acc += (ok ? cv[u].x : 0.f) * xs[lo & 0xffffu];
+            acc += (ok ? cv[u].y : 0.f) * xs[lo >> 16];
+            acc += (ok ? cv[u].z : 0.f) * xs[hi & 0xffffu];
+            acc += (ok ? cv[u].w : 0.f) * xs[hi >> 16];
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            cv[u] = nv[u];
+            ci[u] = ni[u];
+        }
+    }
+    for (int m = 1; m < (1 << glog); m <<= 1) acc += __shfl_xor(acc, m, 64);
+    const int i = wave * (64 >> glog) + (lane >> glog);
+    if ((lane & ((1 << glog) - 1)) == 0 && i < hs) spy[map[i]] += acc;
+}
+
 template <int WG, int UNROLL, int XCAP>
-__device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *xs, float *red, float *prod)
+__device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *xs, float *red, float *spy)
 {
     const int tid = threadIdx.x;
     sb_clock(a.tile_clock, t, 0);
@@ -255,7 +321,6 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
     const int M = tile.dense_mcols;
     const float *__restrict__ x = a.xvec ? a.xvec[tile.mat] : a.x0;
     float *__restrict__ y = a.yvec ? a.yvec[tile.mat] : a.y0;
-    const int chunk = tile.sp_chunk > 0 && tile.sp_chunk <= SB_SPCHUNK ? tile.sp_chunk : SB_SPCHUNK;
 
     SbStreamLane L;
     {
@@ -266,8 +331,8 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         L.c3 = (p0 + 3) / hs;
     }
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    float ysum = 0.f;
     sb_f4 cur[UNROLL];
+    if (tid < hs) spy[tid] = 0.f;   // before the first staging barrier; sparse row owners add after it
 
     int pb = tile.piece_begin;
     while (pb < tile.piece_end) {
@@ -280,7 +345,7 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         const bool pre = first.kind == SB_PIECE_DENSE && tid < tpar;
         if (pre) sb_dense_load<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + first.data_off),
                                        (hs * first.ncols + 3) >> 2, tid, tpar, cur);
-        __syncthreads();                       // previous group done with xs / prod
+        __syncthreads();                       // previous group done with xs
         sb_stage_group<WG>(a.pieces, a.xr, pb, pe, x, xs);
         __syncthreads();
         int base = 0;
@@ -291,23 +356,8 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
                                        (hs * pc.ncols + 3) >> 2, tpar, M, pc.ncols, xs + base, L, cur, pre && p == pb,
                                        a0, a1, a2, a3);
             } else {
-                const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
-                const float *__restrict__ sv = a.sval + pc.data_off;
-                const uint16_t *__restrict__ si = a.sidx + pc.data_off;
-                const int nnzp = rp[hs];
-                const int rb = tid < hs ? rp[tid] : 0;
-                const int re = tid < hs ? rp[tid + 1] : 0;
-                for (int e0 = 0; e0 < nnzp; e0 += chunk) {
-                    const int n = min(chunk, nnzp - e0);
-                    __syncthreads();           // prod reuse
-#pragma unroll 4
-                    for (int e = tid; e < n; e += WG) prod[e] = sv[e0 + e] * xs[base + si[e0 + e]];
-                    __syncthreads();
-                    if (tid < hs) {
-                        const int lo = max(rb, e0), hi = min(re, e0 + n);
-                        for (int e = lo; e < hi; ++e) ysum += prod[e - e0];
-                    }
-                }
+                sb_sparse_piece<SB_SPUNROLL>(a.sval + pc.data_off, a.sidx + pc.data_off, a.srowptr + pc.rowptr_off,
+                                             a.srowptr + tile.sp_map, hs, tile.sp_glog, pc.ncols, xs + base, spy);
             }
             base += pc.ncols + 4;
         }
@@ -335,7 +385,7 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
     if (tid < hs) {
         float s = 0.f;
         for (int g = 0; g < m; ++g) s += red[g * hs + tid];
-        y[tile.row0 + tid] = s + ysum;
+        y[tile.row0 + tid] = s + spy[tid];
     }
     if (a.tile_clock) {
         __syncthreads();
@@ -348,10 +398,10 @@ __device__ __forceinline__ void sb_spmv_body(const SbSpmvArgs &a)
 {
     __shared__ float xs[XCAP + 4];
     __shared__ float red[4 * WG];
-    __shared__ float prod[SB_SPCHUNK];
+    __shared__ float spy[WG];
     __shared__ int next_tile;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmv_tile<WG, UNROLL, XCAP>(a, t, xs, red, prod); });
+                 [&](int t) { sb_spmv_tile<WG, UNROLL, XCAP>(a, t, xs, red, spy); });
 }
 
 // ---------------------------------------------------------------------------

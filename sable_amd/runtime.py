@@ -39,6 +39,15 @@ def device_info():
     return {"n_cu": n.value, "hbm_bytes": hbm.value, "arch": arch.value.decode()}
 
 
+def bandwidth_probe(nbytes=2 << 30, reps=5):
+    """Achievable streaming read and copy rates of the device, GB/s."""
+    init()
+    r = C.c_float()
+    c = C.c_float()
+    check(lib.sable_bandwidth_probe(int(nbytes), reps, C.byref(r), C.byref(c)))
+    return {"read_gbs": r.value, "copy_gbs": c.value}
+
+
 def _vbr_struct(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre):
     keep = {
         "rpntr": np.ascontiguousarray(rpntr, dtype=np.int32),
@@ -141,7 +150,7 @@ class Plan:
 
     def __init__(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, routes=None, *,
                  op="spmv", n_rhs=1, host_only=False, wg=256, xcap=4095, tile_bytes=0,
-                 max_seg_rows=0, min_seg_rows=0, sp_chunk=0, mode=_lib.MODE_QUEUE,
+                 max_seg_rows=0, min_seg_rows=0, mode=_lib.MODE_QUEUE,
                  grid=0, unroll=4, use_rtc=True, mfma_split=3):
         if not host_only:
             init()
@@ -150,7 +159,7 @@ class Plan:
         o.op = _lib.OP_SPMM if op == "spmm" else _lib.OP_SPMV
         o.n_rhs = n_rhs if op == "spmm" else 1
         o.wg, o.xcap, o.tile_bytes = wg, xcap, tile_bytes
-        o.max_seg_rows, o.min_seg_rows, o.sp_chunk = max_seg_rows, min_seg_rows, sp_chunk
+        o.max_seg_rows, o.min_seg_rows = max_seg_rows, min_seg_rows
         o.mode, o.grid, o.unroll = mode, grid, unroll
         o.use_rtc, o.host_only = int(bool(use_rtc)), int(bool(host_only))
         o.mfma_split = int(mfma_split)
@@ -193,7 +202,8 @@ class Plan:
                "srowptr": (6, np.int32)}
     TILE_DTYPE = np.dtype([("row0", "<i4"), ("nrows", "<i4"), ("piece_begin", "<i4"),
                            ("piece_end", "<i4"), ("mat", "<i4"), ("dense_tpar", "<i4"),
-                           ("dense_mcols", "<i4"), ("sp_chunk", "<i4")])
+                           ("dense_mcols", "<i4"), ("sp_glog", "<i4"),
+                           ("sp_map", "<i4")])
     PIECE_DTYPE = np.dtype([("data_off", "<i8"), ("kind", "<i4"), ("ncols", "<i4"),
                             ("xr_begin", "<i4"), ("xr_end", "<i4"), ("rowptr_off", "<i4"),
                             ("ksteps", "<i4")])

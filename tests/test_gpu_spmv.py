@@ -136,15 +136,32 @@ def test_edge_shapes(rt):
     assert np.all(y == 0)
 
 
-@pytest.mark.parametrize("chunk", [1, 7, 256, 1024])
-def test_sparse_chunk_sizes(rt, chunk):
-    """Products staged through LDS in chunks: rows that straddle chunk
-    boundaries (chunk sizes 1 and 7 force it) still sum in CSR order."""
+@pytest.mark.parametrize("wg,seg", [(64, 64), (128, 3), (256, 256), (256, 8), (256, 1), (512, 512)])
+def test_sparse_wave_slices(rt, wg, seg):
+    """Sparse rows in wave slices: tile heights from 1 row to the workgroup
+    size give every lanes-per-row choice from 64 down to 1, with ragged rows
+    (per-row lengths vary with the block density) and 1-4 wavefronts."""
     vbr, x, _ = synth.workload_like_c2(4000, seed=13, frac_blocks=0.03)
     routes = np.full(len(vbr.bindx), ROUTE_SPARSE, np.uint8)
-    print(f"case: sparse chunk {chunk}", flush=True)
-    y = rt.Plan.from_vbr(vbr, routes, sp_chunk=chunk).spmv(x)
+    print(f"case: sparse slices wg={wg} max_seg_rows={seg}", flush=True)
+    plan = rt.Plan.from_vbr(vbr, routes, wg=wg, max_seg_rows=seg)
+    glogs = {int(t["sp_glog"]) for t in plan.export("tiles") if t["sp_map"] >= 0}
+    y = plan.spmv(x)
     assert _err(vbr, x, y)["componentwise"] <= TOL
+    assert glogs  # sparse tiles exist
+
+
+def test_sparse_slices_repeatable_and_mixed(rt):
+    """Dense and sparse pieces in one tile, several sparse groups per tile
+    (small staging capacity): results are bit-identical across launches."""
+    vbr, x, _ = synth.workload_like_c2(3000, seed=17, frac_blocks=0.05)
+    routes = threshold_routes(vbr.val, vbr.indx, 15)
+    print("case: mixed slices xcap=64", flush=True)
+    plan = rt.Plan.from_vbr(vbr, routes, xcap=64)
+    y1 = plan.spmv(x)
+    y2 = plan.spmv(x)
+    assert np.array_equal(y1, y2)
+    assert _err(vbr, x, y1)["componentwise"] <= TOL
 
 
 def test_device_tensors_and_timing(rt):

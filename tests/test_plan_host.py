@@ -27,6 +27,42 @@ def bf16_to_f32(h):
     return (h.astype(np.uint32) << 16).view(np.float32)
 
 
+def slice_rows(t, p, srow, sval, sidx, nw):
+    """Decodes one SpMV wave-sliced sparse piece into {local row: (cols, vals)}
+    and checks the layout: every row of the tile appears once in the row
+    order, each row's entries are a prefix of its lane chunks, and all other
+    slots are padding (value 0 at the zero pad column ncols)."""
+    hs, glog = int(t["nrows"]), int(t["sp_glog"])
+    G, C = 1 << glog, 64 >> glog
+    assert C * nw >= hs
+    order = srow[t["sp_map"]:t["sp_map"] + hs]
+    assert sorted(order.tolist()) == list(range(hs))
+    tab = srow[p["rowptr_off"]:p["rowptr_off"] + 2 * nw].reshape(nw, 2)
+    off = int(p["data_off"])
+    assert off % 256 == 0
+    assert np.array_equal(tab[:, 0], np.concatenate([[0], np.cumsum(tab[:-1, 1])]))
+    ncols = int(p["ncols"])
+    rows = {}
+    for w in range(nw):
+        o, W = int(tab[w, 0]), int(tab[w, 1])
+        blk_v = sval[off + o * 256: off + (o + W) * 256].reshape(W, 64, 4)
+        blk_i = sidx[off + o * 256: off + (o + W) * 256].reshape(W, 64, 4)
+        for q in range(C):
+            lanes = slice(q * G, q * G + G)
+            # chunk c = j*G + g -> entries in row order
+            v = blk_v[:, lanes, :].reshape(-1)
+            c = blk_i[:, lanes, :].reshape(-1)
+            real = c < ncols
+            n = int(real.sum())
+            assert np.all(real[:n]) and not np.any(real[n:]), "entries must be a prefix"
+            assert np.all(v[n:] == 0) and np.all(c[n:] == ncols)
+            if w * C + q < hs:
+                rows[int(order[w * C + q])] = (c[:n].astype(np.int64), v[:n])
+            else:
+                assert n == 0
+    return rows
+
+
 def reconstruct(plan):
     tiles = plan.export("tiles")
     pieces = plan.export("pieces")
@@ -49,6 +85,10 @@ def reconstruct(plan):
                 assert off % 4 == 0
                 blk = dval[off:off + hs * len(cols)].reshape(len(cols), hs).T
                 M[r0:r0 + hs][:, cols] += blk
+            elif p["kind"] == 1 and plan.op == "spmv":
+                for r, (lc, v) in slice_rows(t, p, srow, sval, sidx, plan.stats["wg"] // 64).items():
+                    assert np.all(np.diff(lc) > 0)  # ascending columns in a row
+                    M[r0 + r, cols[lc]] += v
             elif p["kind"] == 1:
                 rp = srow[p["rowptr_off"]:p["rowptr_off"] + hs + 1]
                 for r in range(hs):
@@ -101,7 +141,8 @@ def test_host_plan_launch_invariants():
         hs, tp, M = int(t["nrows"]), int(t["dense_tpar"]), int(t["dense_mcols"])
         assert 1 <= hs <= 256
         assert 4 * tp == M * hs and 0 < tp <= 256
-        assert 0 < t["sp_chunk"] <= 1024
+        if t["sp_map"] >= 0:
+            assert 0 <= t["sp_glog"] <= 6 and (64 >> int(t["sp_glog"])) * 4 >= hs
     for p in plan.export("pieces"):
         assert p["ncols"] <= 4096
 
