@@ -99,6 +99,7 @@ typedef struct {
     int64_t sparse_padded;    /* sparse slots as laid out (SpMV wave slices pad rows) */
     int64_t rowptr_entries;   /* sparse index entries: CSR row pointers (SpMM) or
                                  row orders and wavefront tables (SpMV) */
+    int64_t xmap_entries;     /* SpMV x staging map entries (x column per staged slot) */
     int64_t descriptor_bytes; /* tiles + pieces + ranges */
     int64_t bytes_alg;        /* compulsory bytes of the executed format */
     int64_t flops_alg;        /* 2 * nnz * N */
@@ -126,7 +127,7 @@ int sable_plan_stats(const SablePlan *plan, SablePlanStats *out);
 int sable_plan_set_stream(SablePlan *plan, void *hip_stream);
 /* Host copies of the plan's arrays (for inspection / tests).  which: 0 tiles,
  * 1 pieces, 2 xranges, 3 dval (FP32 dense and MFMA pieces), 4 sval, 5 sidx,
- * 6 srowptr.
+ * 6 srowptr, 7 xmap (SpMV).
  * Returns the byte size in *bytes; copies when dst != NULL. */
 int sable_plan_export(const SablePlan *plan, int which, void *dst, int64_t *bytes);
 /* The kernel specialisation chosen for the plan (no reference counterpart):

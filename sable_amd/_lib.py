@@ -58,7 +58,7 @@ class SablePlanStats(C.Structure):
                 ("nblocks_sparse", I64), ("nblocks_mfma", I64), ("nnz", I64),
                 ("dense_values", I64), ("dense_padded", I64), ("mfma_values", I64),
                 ("mfma_padded", I64), ("sparse_nnz", I64), ("sparse_padded", I64),
-                ("rowptr_entries", I64),
+                ("rowptr_entries", I64), ("xmap_entries", I64),
                 ("descriptor_bytes", I64), ("bytes_alg", I64), ("flops_alg", I64),
                 ("device_bytes", I64), ("rtc_used", I32), ("mode", I32), ("grid", I32),
                 ("wg", I32)]

@@ -138,6 +138,29 @@ void dense_stream_params(int hs, int wg, int32_t &tpar, int32_t &mcols)
 
 }  // namespace
 
+// SpMV staging of one tile: consecutive pieces are grouped while their slices
+// (ncols + 4 zero pads each) fit SB_STAGE_CAP + 4 slots, and each piece's
+// slice is described slot by slot in xmap (x column, -1 for a pad), pieces
+// of the tile back to back, so a group's map is one contiguous run.
+void stage_groups(HostPlan &P, const SbTile &t)
+{
+    int32_t p = t.piece_begin;
+    while (p < t.piece_end) {
+        int32_t q = p, used = 0;
+        while (q < t.piece_end && used + P.pieces[q].ncols + 4 <= SB_STAGE_CAP + 4) used += P.pieces[q++].ncols + 4;
+        P.pieces[p].grp_end = q;
+        P.pieces[p].stage_len = used;
+        p = q;
+    }
+    for (int32_t k = t.piece_begin; k < t.piece_end; ++k) {
+        SbPiece &pc = P.pieces[k];
+        pc.xmap_off = (int32_t)P.xmap.size();
+        for (int32_t r = pc.xr_begin; r < pc.xr_end; ++r)
+            for (int32_t c = 0; c < P.xr[r].len; ++c) P.xmap.push_back(P.xr[r].col0 + c);
+        P.xmap.insert(P.xmap.end(), 4, -1);
+    }
+}
+
 // Nonzeros of rows [s0, s0+hs) of a block row (height h) restricted to one
 // column group, row by row in column order (local column = position in the
 // group's concatenated columns).  Returns false when there are none.
@@ -472,6 +495,7 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                 }
             }
             t.piece_end = (int32_t)P.pieces.size();
+            if (!spmm) stage_groups(P, t);
             if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
             P.tiles.push_back(t);
             P.tile_cost.push_back(cost);
@@ -496,8 +520,11 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
     S.nxranges = (int32_t)P.xr.size();
     S.dense_padded = (int64_t)P.dval.size() - S.mfma_padded;
     S.sparse_padded = (int64_t)P.sval.size();
-    S.descriptor_bytes = (int64_t)(P.tiles.size() * sizeof(SbTile) + P.pieces.size() * sizeof(SbPiece) +
-                                   P.xr.size() * sizeof(SbXRange));
+    if (P.xmap.size() > (size_t)INT32_MAX) { err = "x staging map exceeds 2^31 entries"; return SABLE_ERR_UNSUPPORTED; }
+    S.xmap_entries = (int64_t)P.xmap.size();
+    // what the kernels read: SpMV stages x through xmap, SpMM through the ranges
+    S.descriptor_bytes = (int64_t)(P.tiles.size() * sizeof(SbTile) + P.pieces.size() * sizeof(SbPiece)) +
+                         (spmm ? (int64_t)(P.xr.size() * sizeof(SbXRange)) : 4 * S.xmap_entries);
     S.bytes_alg = 4 * S.dense_values + 4 * S.mfma_values + 6 * S.sparse_nnz + 4 * S.rowptr_entries +
                   S.descriptor_bytes + 4LL * P.ncols * P.n_rhs + 4LL * P.nrows * P.n_rhs;
     S.flops_alg = 2 * S.nnz * (int64_t)P.n_rhs;

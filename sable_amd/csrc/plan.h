@@ -22,9 +22,14 @@ struct HostPlan {
     std::vector<float> sval;
     std::vector<uint16_t> sidx;
     std::vector<int32_t> srowptr;
+    std::vector<int32_t> xmap;    // SpMV: x column per staged slot (-1 = zero pad)
     std::vector<double> tile_cost;   // estimated bytes per tile (ordering)
     SablePlanStats stats{};
 };
+
+// SpMV x staging capacity the host groups pieces for: a group's staged slots
+// (ncols + 4 per piece) never exceed SB_STAGE_CAP + 4.
+constexpr int SB_STAGE_CAP = 4096;
 
 // Fills defaults for zero / negative fields.
 SablePlanOpts normalize_opts(const SablePlanOpts *in);

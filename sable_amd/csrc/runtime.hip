@@ -294,6 +294,7 @@ struct SablePlan {
     float *d_sval = nullptr;
     uint16_t *d_sidx = nullptr;
     int32_t *d_srowptr = nullptr;
+    int32_t *d_xmap = nullptr;
     int32_t *d_queue = nullptr;
     const float **d_xptr = nullptr;
     float **d_yptr = nullptr;
@@ -324,6 +325,7 @@ struct SableGroup {
     float *d_sval = nullptr;
     uint16_t *d_sidx = nullptr;
     int32_t *d_srowptr = nullptr;
+    int32_t *d_xmap = nullptr;
     int32_t *d_queue = nullptr;
     const float **d_xptr = nullptr;
     float **d_yptr = nullptr;
@@ -364,16 +366,11 @@ int max_piece_cols(const sable::HostPlan &h)
     return m;
 }
 
-// LDS x buffer a SpMV tile needs to stage all its pieces at once
-// (each slice followed by 4 zero pads).
-int max_tile_staging(const sable::HostPlan &h)
+// x buffer (slots beyond the 4 trailing pads) the largest SpMV staging group needs.
+int max_group_staging(const sable::HostPlan &h)
 {
     int m = 1;
-    for (auto &t : h.tiles) {
-        int s = 0;
-        for (int p = t.piece_begin; p < t.piece_end; ++p) s += h.pieces[p].ncols + 4;
-        m = std::max(m, s);
-    }
+    for (auto &p : h.pieces) m = std::max(m, p.stage_len - 4);
     return m;
 }
 
@@ -394,11 +391,10 @@ int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, h
     s.wg = o.wg;
     s.unroll = o.unroll;
     const bool spmm = h.op == SABLE_OP_SPMM;
-    // SpMV: room for a whole tile's slices when possible; never below the
-    // widest piece (the kernel needs every piece to fit on its own)
+    // SpMV: room for the largest staging group the host formed (at most
+    // SB_STAGE_CAP + 4 slots, so the buffer never exceeds 4096 + 4)
     s.xcap = spmm ? std::min(4096, pow2_at_least(max_piece_cols(h) + 1))
-                  : std::min(4096, std::max(pow2_at_least(max_tile_staging(h)),
-                                            pow2_at_least(max_piece_cols(h))));
+                  : std::min(sable::SB_STAGE_CAP, std::max(16, pow2_at_least(max_group_staging(h))));
     const bool mfma = spmm && has_kind(h, SB_PIECE_MFMA);
     s.split = o.mfma_split == 2 ? 2 : 3;
     if (spmm) {
@@ -472,6 +468,7 @@ int plan_launch(SablePlan *p, const float *x, float *y)
         SbSpmvArgs a{};
         a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
         a.dval = p->d_dval; a.sval = p->d_sval; a.sidx = p->d_sidx; a.srowptr = p->d_srowptr;
+        a.xmap = p->d_xmap;
         a.xvec = nullptr; a.yvec = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
         a.tile_clock = p->clock_on ? p->d_clock : nullptr;
         a.ntiles = ntiles; a.mode = mode;
@@ -610,6 +607,7 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     CHECKRC(upload(&p->d_sval, h.sval, bytes));
     CHECKRC(upload(&p->d_sidx, h.sidx, bytes));
     CHECKRC(upload(&p->d_srowptr, h.srowptr, bytes));
+    CHECKRC(upload(&p->d_xmap, h.xmap, bytes));
     HIPOK(hipMalloc((void **)&p->d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(p->d_queue, 0, 2 * sizeof(int32_t)));
     HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
@@ -624,6 +622,7 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     p->host.sval = std::vector<float>();
     p->host.sidx = std::vector<uint16_t>();
     p->host.srowptr = std::vector<int32_t>();
+    p->host.xmap = std::vector<int32_t>();
     p->keep_host = false;
     *out = p.release();
     return SABLE_OK;
@@ -633,7 +632,7 @@ int sable_plan_destroy(SablePlan *p)
 {
     if (!p) return SABLE_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
-    void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr,
+    void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap,
                     p->d_queue, p->d_xptr, p->d_yptr, p->d_clock, p->d_xtmp, p->d_ytmp, p->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -681,6 +680,7 @@ int sable_plan_export(const SablePlan *p, int which, void *dst, int64_t *bytes)
     case 4: hsrc = h.sval.data(); dsrc = p->d_sval; n = p->stats.sparse_padded * 4; break;
     case 5: hsrc = h.sidx.data(); dsrc = p->d_sidx; n = p->stats.sparse_padded * 2; break;
     case 6: hsrc = h.srowptr.data(); dsrc = p->d_srowptr; n = p->stats.rowptr_entries * 4; break;
+    case 7: hsrc = h.xmap.data(); dsrc = p->d_xmap; n = p->stats.xmap_entries * 4; break;
     default: return fail(SABLE_ERR_INVALID, "unknown export id");
     }
     *bytes = n;
@@ -809,7 +809,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     std::vector<SbPiece> pieces;
     std::vector<SbXRange> xr;
     std::vector<double> cost;
-    int64_t dval_n = 0, sval_n = 0, srow_n = 0;
+    int64_t dval_n = 0, sval_n = 0, srow_n = 0, xmap_n = 0;
     int maxcols = 1, wg = plans[0]->opts.wg;
     bool mfma = false;
     for (int i = 0; i < nplans; ++i) {
@@ -828,6 +828,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
                 pc.data_off += sval_n;   // multiples of 256 entries keep SpMV float4 alignment
                 pc.rowptr_off += (int32_t)srow_n;
             }
+            pc.xmap_off += (int32_t)xmap_n;
             maxcols = std::max(maxcols, pc.ncols);
             pieces.push_back(pc);
         }
@@ -844,7 +845,9 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
         dval_n += p->stats.dense_padded + p->stats.mfma_padded;
         sval_n += p->stats.sparse_padded;
         srow_n += p->stats.rowptr_entries;
+        xmap_n += p->stats.xmap_entries;
     }
+    if (xmap_n > INT32_MAX) return fail(SABLE_ERR_UNSUPPORTED, "group x staging map exceeds 2^31 entries");
     // global LPT order
     std::vector<int32_t> order(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
@@ -876,6 +879,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     CHECKRC(cat(&g->d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_padded); }, 4));
     CHECKRC(cat(&g->d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_padded); }, 2));
     CHECKRC(cat(&g->d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
+    CHECKRC(cat(&g->d_xmap, xmap_n, [](SablePlan *p) { return P2(p->d_xmap, p->stats.xmap_entries); }, 4));
     HIPOK(hipMalloc((void **)&g->d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(g->d_queue, 0, 2 * sizeof(int32_t)));
     HIPOK(hipMalloc((void **)&g->d_xptr, nplans * sizeof(float *)));
@@ -900,7 +904,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     if (g->op == SABLE_OP_SPMV) {
         SbSpmvArgs a{};
         a.tiles = g->d_tiles; a.pieces = g->d_pieces; a.xr = g->d_xr; a.dval = g->d_dval;
-        a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr;
+        a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr; a.xmap = g->d_xmap;
         a.xvec = g->d_xptr; a.yvec = g->d_yptr; a.queue = g->d_queue;
         a.ntiles = g->ntiles; a.mode = g->mode;
         rc = launch_raw(g->rtc, g->fn, g->aot, g->grid, wg, &a, g->stream);
@@ -962,7 +966,7 @@ int sable_group_destroy(SableGroup *g)
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     void *bufs[] = {g->d_tiles, g->d_pieces, g->d_xr, g->d_dval, g->d_sval, g->d_sidx, g->d_srowptr,
-                    g->d_queue, g->d_xptr, g->d_yptr, g->d_flush};
+                    g->d_xmap, g->d_queue, g->d_xptr, g->d_yptr, g->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (g->stream) hipStreamDestroy(g->stream);

@@ -47,6 +47,14 @@ struct SbPiece {
                              // to data_off); SpMV: per-wavefront {step offset,
                              // steps} pairs at srowptr[rowptr_off ..]
     int32_t ksteps;          // mfma: number of 32-wide k steps (ncols padded)
+    // SpMV staging: pieces are staged in groups of consecutive pieces whose
+    // slices (ncols + 4 zero pads each) fit the x buffer together.
+    int32_t grp_end;         // first piece of a group: one past its last piece; else 0
+    int32_t stage_len;       // first piece of a group: staged slots of the group
+    int32_t xmap_off;        // xmap[xmap_off .. + ncols + 4): x column of each staged
+                             // slot, -1 for the zero pads; consecutive pieces of a
+                             // tile are consecutive in xmap
+    int32_t reserved;
 };
 
 // A contiguous run of x (or X rows) copied into the staging buffer.

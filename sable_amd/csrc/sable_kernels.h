@@ -43,6 +43,7 @@ struct SbSpmvArgs {
     const float *sval;
     const uint16_t *sidx;
     const int32_t *srowptr;
+    const int32_t *xmap;          // x column per staged slot (-1: zero pad)
     const float *const *xvec;     // per matrix slot (batched plans) or null
     float *const *yvec;           // per matrix slot (batched plans) or null
     const float *x0;              // single plan: x
@@ -212,45 +213,28 @@ __device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int 
     }
 }
 
-// Stages the x slices of pieces [pb, pe) into xs: piece p at base_p, each
-// slice followed by 4 zeros.  One flattened pass, 4 loads in flight per thread.
+// Stages a group's x slices into xs through its slot map (x column per slot,
+// -1 for the zero pads): two dependent loads per slot, 4 slots in flight per
+// thread, no search over descriptors.
 template <int WG>
-__device__ __forceinline__ void sb_stage_group(const SbPiece *__restrict__ pieces, const SbXRange *__restrict__ xr,
-                                               int pb, int pe, const float *__restrict__ x, float *xs)
+__device__ __forceinline__ void sb_stage_map(const int32_t *__restrict__ map, int total,
+                                             const float *__restrict__ x, float *xs)
 {
-    int total = 0;
-    for (int p = pb; p < pe; ++p) total += pieces[p].ncols + 4;
     for (int i0 = 0; i0 < total; i0 += 4 * WG) {
-        float v[4];
-        int dst[4];
+        int m[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * WG + threadIdx.x;
-            dst[u] = -1;
-            v[u] = 0.f;
-            if (i < total) {
-                dst[u] = i;
-                int base = 0, p = pb;
-                while (i >= base + pieces[p].ncols + 4) {
-                    base += pieces[p].ncols + 4;
-                    ++p;
-                }
-                int off = i - base;
-                if (off < pieces[p].ncols) {
-                    for (int r = pieces[p].xr_begin; r < pieces[p].xr_end; ++r) {
-                        const SbXRange g = xr[r];
-                        if (off < g.len) {
-                            v[u] = x[g.col0 + off];
-                            break;
-                        }
-                        off -= g.len;
-                    }
-                }
-            }
+            const int i = i0 + u * WG + (int)threadIdx.x;
+            m[u] = i < total ? __builtin_nontemporal_load(&map[i]) : -1;
         }
+        float v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (dst[u] >= 0) xs[dst[u]] = v[u];
+        for (int u = 0; u < 4; ++u) v[u] = m[u] >= 0 ? x[m[u]] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * WG + (int)threadIdx.x;
+            if (i < total) xs[i] = v[u];
+        }
     }
 }
 
@@ -336,17 +320,16 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
 
     int pb = tile.piece_begin;
     while (pb < tile.piece_end) {
-        // staging group [pb, pe)
-        int pe = pb, used = 0;
-        while (pe < tile.piece_end && used + a.pieces[pe].ncols + 4 <= XCAP + 4) used += a.pieces[pe++].ncols + 4;
-        if (pe == pb) break;   // a piece wider than the buffer: never built (the host checks)
-        // issue the first dense batch before staging (it does not need x)
+        // staging group [pb, pe), formed by the host (stage_len <= XCAP + 4 slots)
         const SbPiece first = a.pieces[pb];
+        const int pe = first.grp_end;
+        if (pe <= pb || first.stage_len > XCAP + 4) break;   // never built (the host and kernel choice check)
+        // issue the first dense batch before staging (it does not need x)
         const bool pre = first.kind == SB_PIECE_DENSE && tid < tpar;
         if (pre) sb_dense_load<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + first.data_off),
                                        (hs * first.ncols + 3) >> 2, tid, tpar, cur);
         __syncthreads();                       // previous group done with xs
-        sb_stage_group<WG>(a.pieces, a.xr, pb, pe, x, xs);
+        sb_stage_map<WG>(a.xmap + first.xmap_off, first.stage_len, x, xs);
         __syncthreads();
         int base = 0;
         for (int p = pb; p < pe; ++p) {

@@ -199,14 +199,15 @@ class Plan:
     # -- inspection -----------------------------------------------------
     _EXPORT = {"tiles": (0, None), "pieces": (1, None), "xranges": (2, np.int32),
                "dval": (3, np.float32), "sval": (4, np.float32), "sidx": (5, np.uint16),
-               "srowptr": (6, np.int32)}
+               "srowptr": (6, np.int32), "xmap": (7, np.int32)}
     TILE_DTYPE = np.dtype([("row0", "<i4"), ("nrows", "<i4"), ("piece_begin", "<i4"),
                            ("piece_end", "<i4"), ("mat", "<i4"), ("dense_tpar", "<i4"),
                            ("dense_mcols", "<i4"), ("sp_glog", "<i4"),
                            ("sp_map", "<i4")])
     PIECE_DTYPE = np.dtype([("data_off", "<i8"), ("kind", "<i4"), ("ncols", "<i4"),
                             ("xr_begin", "<i4"), ("xr_end", "<i4"), ("rowptr_off", "<i4"),
-                            ("ksteps", "<i4")])
+                            ("ksteps", "<i4"), ("grp_end", "<i4"), ("stage_len", "<i4"),
+                            ("xmap_off", "<i4"), ("reserved", "<i4")])
 
     def export(self, name):
         which, dt = self._EXPORT[name]

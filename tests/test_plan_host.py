@@ -145,7 +145,32 @@ def test_host_plan_launch_invariants():
             assert 0 <= t["sp_glog"] <= 6 and (64 >> int(t["sp_glog"])) * 4 >= hs
     for p in plan.export("pieces"):
         assert p["ncols"] <= 4096
+    check_staging(plan)
 
+
+def check_staging(plan):
+    """SpMV staging: every tile's pieces are covered by consecutive groups of
+    at most 4096 + 4 slots, and the slot map of each piece is its x ranges
+    followed by four -1 pads, pieces of a tile back to back."""
+    tiles, pieces = plan.export("tiles"), plan.export("pieces")
+    xr, xmap = plan.export("xranges"), plan.export("xmap")
+    assert plan.stats["xmap_entries"] == len(xmap)
+    for t in tiles:
+        p = int(t["piece_begin"])
+        while p < t["piece_end"]:
+            q = int(pieces[p]["grp_end"])
+            assert p < q <= t["piece_end"]
+            slots = sum(int(pieces[k]["ncols"]) + 4 for k in range(p, q))
+            assert pieces[p]["stage_len"] == slots <= 4100
+            for k in range(p + 1, q):
+                assert pieces[k]["grp_end"] == 0
+                assert pieces[k]["xmap_off"] == pieces[k - 1]["xmap_off"] + pieces[k - 1]["ncols"] + 4
+            p = q
+        for k in range(t["piece_begin"], t["piece_end"]):
+            pc = pieces[k]
+            cols = np.concatenate([np.arange(c0, c0 + n) for c0, n in xr[pc["xr_begin"]:pc["xr_end"]]])
+            m = xmap[pc["xmap_off"]:pc["xmap_off"] + pc["ncols"] + 4]
+            assert np.array_equal(m[:-4], cols) and np.all(m[-4:] == -1)
 
 def test_host_plan_splits_wide_and_tall_blocks():
     """Blocks wider than the staging capacity are split by columns, block
@@ -159,6 +184,7 @@ def test_host_plan_splits_wide_and_tall_blocks():
     for route in (ROUTE_DENSE, ROUTE_SPARSE):
         plan = Plan(*args, routes=[route], host_only=True, xcap=16)
         M, _, cover = reconstruct(plan)
+        check_staging(plan)
         assert np.array_equal(M, val.reshape(w, h).T.astype(np.float64))
         assert np.all(cover == 1)
         assert all(p["ncols"] <= 16 for p in plan.export("pieces"))
