@@ -146,9 +146,9 @@ int sable_spmv(SablePlan *plan, const float *x, float *y, int flags);
 int sable_spmm(SablePlan *plan, const float *X, float *Y, int flags);
 
 /* Replaces: the timed loop of src/codegen.py:414-439 and eval.py:23-45.
- * Runs `warmup` untimed launches then `reps` launches, each bracketed by
- * HIP events on the plan's stream; times_ms[r] receives each launch's
- * duration.  When flush_bytes > 0 a buffer of that size is overwritten
+ * Runs `warmup` untimed launches then `reps` launches, each timed by HIP
+ * events attached to the kernel launch itself (its start and end, without
+ * the dispatch latency); times_ms[r] receives each launch's duration.  When flush_bytes > 0 a buffer of that size is overwritten
  * before every launch (outside the events) so each launch starts with cold
  * caches.  x/y (or X/Y) are device pointers. */
 int sable_time(SablePlan *plan, const float *x, float *y, int warmup, int reps,

@@ -1,7 +1,9 @@
 """ctypes binding of libsable_amd.so (declarations mirror include/sable.h).
 
 Importing this module loads the native library and raises ``ImportError``
-when it has not been built: the GPU path has no CPU fallback.
+when it has not been built: the GPU path has no CPU fallback.  The
+environment variable ``SABLE_AMD_LIBRARY`` names another build of the
+library to load instead (same C-ABI).
 """
 from __future__ import annotations
 
@@ -9,7 +11,7 @@ import ctypes as C
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libsable_amd.so")
+LIB_PATH = os.environ.get("SABLE_AMD_LIBRARY") or os.path.join(PKG, "libsable_amd.so")
 RTC_CACHE = os.path.join(PKG, "_rtc_cache")
 
 SABLE_OK = 0

@@ -24,10 +24,10 @@ import numpy as np
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 SPMV_SPACE = {
-    "unroll": (2, 4, 8),
-    "tile_bytes": (64 << 10, 128 << 10, 256 << 10, 512 << 10, 1 << 20),
-    "mode": (0, 2),
-    "wg": (256,),
+    "unroll": (2, 4),
+    "tile_bytes": (128 << 10, 256 << 10, 512 << 10, 1 << 20),
+    "mode": (0,),
+    "wg": (256, 512),
 }
 SPMM_SPACE = {
     "tile_bytes": (32 << 10, 64 << 10, 128 << 10),

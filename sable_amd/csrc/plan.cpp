@@ -138,7 +138,8 @@ void dense_stream_params(int hs, int wg, int32_t &tpar, int32_t &mcols)
 
 }  // namespace
 
-// SpMV staging of one tile: consecutive pieces are grouped while their slices
+// SpMV staging of one tile (pieces already ordered sparse first): consecutive
+// pieces are grouped while their slices
 // (ncols + 4 zero pads each) fit SB_STAGE_CAP + 4 slots, and each piece's
 // slice is described slot by slot in xmap (x column, -1 for a pad), pieces
 // of the tile back to back, so a group's map is one contiguous run.
@@ -207,20 +208,20 @@ bool collect_sparse_rows(const SableVbr &v, const std::vector<ColSeg> &g, int32_
 }
 
 // SpMV sparse layout ("wave slices").  The tile's rows are ordered by total
-// sparse length (longest first, ties by row) and dealt in runs of C = 64/G
-// rows to the wg/64 wavefronts; each row gets G lanes.  A wavefront step is
-// 256 consecutive entries (64 lanes x 4): lane q*G + g of the wavefront
-// takes 4-entry chunk c = j*G + g of its row at step j, so every step is one
-// coalesced 1 KiB value load and 512 B index load per wavefront.  Each
-// wavefront's steps are contiguous; a row is padded to the wavefront's step
-// count with value 0 at the zero pad slot of the staged x slice (local
-// column ncols).  Per piece the table holds {step offset, steps} for each
-// wavefront.  G is chosen per tile to minimise padded slots plus half the
-// slots the longest wavefront keeps the tile busy for.  Returns the cost.
+// sparse length (longest first, ties by row) and cut into slices of C = 64/G
+// consecutive rows; each row gets G lanes of a wavefront.  Slice s belongs
+// to sparse wavefront s % nsw.  A slice step is 256 consecutive entries (64
+// lanes x 4): lane q*G + g takes 4-entry chunk c = j*G + g of its row at step
+// j, so every step is one coalesced 1 KiB value load and 512 B index load.
+// A row is padded to its slice's step count with value 0 at the zero pad
+// slot of the staged x slice (local column ncols).  Per piece the table
+// holds {step offset, steps} for each slice.  G is chosen per tile to
+// minimise padded slots plus half the slots the busiest sparse wavefront
+// keeps the tile waiting for (a slice start counts two steps).  Returns the
+// cost.
 double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::vector<SparseRows> &sp,
-                        const std::vector<std::vector<ColSeg>> &sgroups, int32_t hs, int32_t wg)
+                        const std::vector<std::vector<ColSeg>> &sgroups, int32_t hs, int nsw)
 {
-    const int nw = wg / 64;
     std::vector<int64_t> tot(hs, 0);
     for (auto &sr : sp)
         for (int32_t r = 0; r < hs; ++r) tot[r] += sr.rp[r + 1] - sr.rp[r];
@@ -228,41 +229,41 @@ double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::ve
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return tot[x] > tot[y]; });
 
-    // smallest power-of-two run that covers the rows with nw wavefronts
-    int cmin = 1;
-    while (cmin * nw < hs) cmin <<= 1;
+    // steps of every (piece, slice) for lanes-per-row 2^glog
     auto widths = [&](int glog, std::vector<std::vector<int32_t>> *out) {
         const int G = 1 << glog, C = 64 >> glog;
+        const int nsl = (hs + C - 1) / C;
         int64_t padded = 0, crit = 0;
-        std::vector<int64_t> wsteps(nw, 0);
+        std::vector<int64_t> wsteps(nsw, 0);
         for (size_t p = 0; p < sp.size(); ++p) {
-            std::vector<int32_t> w(nw, 0);
-            for (int wv = 0; wv < nw; ++wv) {
+            std::vector<int32_t> w(nsl, 0);
+            for (int s = 0; s < nsl; ++s) {
                 int32_t m = 0;
-                for (int q = 0; q < C && wv * C + q < hs; ++q) {
-                    const int32_t r = order[wv * C + q];
+                for (int q = 0; q < C && s * C + q < hs; ++q) {
+                    const int32_t r = order[s * C + q];
                     const int32_t len = sp[p].rp[r + 1] - sp[p].rp[r];
                     m = std::max(m, (len + 4 * G - 1) / (4 * G));
                 }
-                w[wv] = m;
-                wsteps[wv] += m;
+                w[s] = m;
+                wsteps[s % nsw] += m + 2;
                 padded += 256LL * m;
             }
             if (out) out->push_back(w);
         }
-        for (int wv = 0; wv < nw; ++wv) crit = std::max(crit, wsteps[wv]);
+        for (int k = 0; k < nsw; ++k) crit = std::max(crit, wsteps[k]);
         return std::make_pair(padded, crit);
     };
     int best = 0;
     double best_cost = 0;
-    for (int glog = 0; (64 >> glog) >= cmin; ++glog) {
+    for (int glog = 0; glog <= 6; ++glog) {
         const auto pc = widths(glog, nullptr);
-        const double c = (double)pc.first + 0.5 * 256.0 * nw * (double)pc.second;
+        const double c = (double)pc.first + 0.5 * 256.0 * nsw * (double)pc.second;
         if (glog == 0 || c < best_cost) { best = glog; best_cost = c; }
     }
     std::vector<std::vector<int32_t>> W;
     widths(best, &W);
     const int G = 1 << best, C = 64 >> best;
+    const int nsl = (hs + C - 1) / C;
     t.sp_glog = best;
     t.sp_map = (int32_t)P.srowptr.size();
     P.srowptr.insert(P.srowptr.end(), order.begin(), order.end());
@@ -276,34 +277,34 @@ double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::ve
         pc.data_off = (int64_t)P.sval.size();
         pc.rowptr_off = (int32_t)P.srowptr.size();
         int64_t steps = 0;
-        for (int wv = 0; wv < nw; ++wv) {
+        for (int s = 0; s < nsl; ++s) {
             P.srowptr.push_back((int32_t)steps);
-            P.srowptr.push_back(W[p][wv]);
-            steps += W[p][wv];
+            P.srowptr.push_back(W[p][s]);
+            steps += W[p][s];
         }
-        S.rowptr_entries += 2 * nw;
+        S.rowptr_entries += 2 * nsl;
         P.sval.resize(P.sval.size() + 256 * steps, 0.0f);
         P.sidx.resize(P.sidx.size() + 256 * steps, (uint16_t)pc.ncols);
         float *dv = P.sval.data() + pc.data_off;
         uint16_t *di = P.sidx.data() + pc.data_off;
-        int64_t woff = 0;
-        for (int wv = 0; wv < nw; ++wv) {
-            for (int q = 0; q < C && wv * C + q < hs; ++q) {
-                const int32_t r = order[wv * C + q];
+        int64_t soff = 0;
+        for (int s = 0; s < nsl; ++s) {
+            for (int q = 0; q < C && s * C + q < hs; ++q) {
+                const int32_t r = order[s * C + q];
                 for (int32_t e = sr.rp[r]; e < sr.rp[r + 1]; ++e) {
                     const int32_t i = e - sr.rp[r];          // position in the row
                     const int32_t c = i / 4, k = i % 4;      // chunk, slot
-                    const int64_t pos = (woff + c / G) * 256 + (int64_t)(q * G + c % G) * 4 + k;
+                    const int64_t pos = (soff + c / G) * 256 + (int64_t)(q * G + c % G) * 4 + k;
                     dv[pos] = sr.val[e];
                     di[pos] = sr.col[e];
                 }
             }
-            woff += W[p][wv];
+            soff += W[p][s];
         }
         S.sparse_nnz += (int64_t)sr.val.size();
         push_xranges(P, sgroups[sr.group], pc);
         P.pieces.push_back(pc);
-        cost += 6.0 * 256.0 * (double)steps + 8.0 * nw;
+        cost += 6.0 * 256.0 * (double)steps + 8.0 * nsl;
     }
     return cost;
 }
@@ -407,7 +408,31 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
             t.row0 = r0 + s0;
             t.nrows = hs;
             t.mat = 0;
-            dense_stream_params(std::min<int32_t>(hs, o.wg), o.wg, t.dense_tpar, t.dense_mcols);
+            // sparse rows of the segment, per sparse column group
+            std::vector<SparseRows> sp;
+            int64_t sp_nnz = 0;
+            for (size_t gi = 0; gi < sgroups.size(); ++gi) {
+                SparseRows sr;
+                if (collect_sparse_rows(v, sgroups[gi], h, s0, hs, sr)) {
+                    sr.group = (int32_t)gi;
+                    sp_nnz += (int64_t)sr.val.size();
+                    sp.push_back(std::move(sr));
+                }
+            }
+            // SpMV with dense and sparse pieces: the last sp_waves wavefronts
+            // take the (latency-bound) sparse pieces while the others stream
+            // the dense ones; the dense stream needs at least hs lanes.
+            const int nw = o.wg / 64;
+            t.sp_waves = 0;
+            if (!spmm && !sp.empty() && !dgroups.empty() && nw >= 2) {
+                const double sb = 6.0 * (double)sp_nnz, db = 4.0 * hs * (double)wd;
+                int k = (int)std::lround(2.0 * nw * sb / (sb + db));
+                k = std::max(1, std::min(k, nw - 1));
+                while (k > 0 && 64 * (nw - k) < hs) --k;
+                t.sp_waves = k;
+            }
+            const int dlanes = 64 * (nw - t.sp_waves);
+            dense_stream_params(std::min<int32_t>(hs, dlanes), dlanes, t.dense_tpar, t.dense_mcols);
             t.piece_begin = (int32_t)P.pieces.size();
             double cost = 64.0;
             // dense FP32 pieces
@@ -461,15 +486,6 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                 P.pieces.push_back(pc);
                 cost += 4.0 * hs * pc.ncols;
             }
-            // sparse pieces: the rows' nonzeros in column order, per group
-            std::vector<SparseRows> sp;
-            for (size_t gi = 0; gi < sgroups.size(); ++gi) {
-                SparseRows sr;
-                if (collect_sparse_rows(v, sgroups[gi], h, s0, hs, sr)) {
-                    sr.group = (int32_t)gi;
-                    sp.push_back(std::move(sr));
-                }
-            }
             t.sp_glog = 0;
             t.sp_map = -1;
             if (!sp.empty()) {
@@ -491,11 +507,16 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
                         cost += 6.0 * tot + 4.0 * (hs + 1);
                     }
                 } else {
-                    cost += emit_wave_slices(P, S, t, sp, sgroups, hs, o.wg);
+                    cost += emit_wave_slices(P, S, t, sp, sgroups, hs, t.sp_waves ? t.sp_waves : nw);
                 }
             }
             t.piece_end = (int32_t)P.pieces.size();
-            if (!spmm) stage_groups(P, t);
+            if (!spmm) {
+                // SpMV: sparse pieces first in every staging group
+                std::stable_partition(P.pieces.begin() + t.piece_begin, P.pieces.begin() + t.piece_end,
+                                      [](const SbPiece &pc) { return pc.kind == SB_PIECE_SPARSE; });
+                stage_groups(P, t);
+            }
             if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
             P.tiles.push_back(t);
             P.tile_cost.push_back(cost);

@@ -6,6 +6,7 @@
 // modes, HIP-event timing with optional cache flushing,
 // graph-captured multi-plan launches, and the router MLP kernels.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -20,6 +21,7 @@
 #include <string>
 #include <vector>
 #include <sys/stat.h>
+#include <sys/time.h>
 #include <unistd.h>
 
 #include "../../include/sable.h"
@@ -202,7 +204,10 @@ int rtc_code_object(const std::string &defines, const std::string &stub, std::ve
         std::ifstream f(path, std::ios::binary);
         if (f) {
             code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
-            if (!code.empty()) return SABLE_OK;
+            if (!code.empty()) {
+                utimes(path.c_str(), nullptr);   // marks the entry as in use (cache pruning)
+                return SABLE_OK;
+            }
         }
     }
     std::string log;
@@ -445,19 +450,28 @@ int occupancy_grid(bool rtc, hipFunction_t fn, const void *aot, int wg, int ntil
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
 }
 
+// Launches one kernel.  With events, e0/e1 take the kernel's own start and
+// end times (AMD launch extension), so timings exclude the dispatch latency.
 int launch_raw(bool rtc, hipFunction_t fn, const void *aot, int grid, int wg, void *argp,
-               hipStream_t st)
+               hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     void *args[] = {argp};
     if (rtc) {
-        HIPOK(hipModuleLaunchKernel(fn, grid, 1, 1, wg, 1, 1, 0, st, args, nullptr));
+        if (e0) {
+            HIPOK(hipExtModuleLaunchKernel(fn, (uint32_t)grid * (uint32_t)wg, 1, 1, wg, 1, 1, 0, st, args, nullptr,
+                                           e0, e1, 0));
+        } else {
+            HIPOK(hipModuleLaunchKernel(fn, grid, 1, 1, wg, 1, 1, 0, st, args, nullptr));
+        }
+    } else if (e0) {
+        HIPOK(hipExtLaunchKernel(aot, dim3(grid), dim3(wg), args, 0, st, e0, e1, 0));
     } else {
         HIPOK(hipLaunchKernel(aot, dim3(grid), dim3(wg), args, 0, st));
     }
     return SABLE_OK;
 }
 
-int plan_launch(SablePlan *p, const float *x, float *y)
+int plan_launch(SablePlan *p, const float *x, float *y, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const sable::HostPlan &h = p->host;
     const int ntiles = p->stats.ntiles;
@@ -472,7 +486,7 @@ int plan_launch(SablePlan *p, const float *x, float *y)
         a.xvec = nullptr; a.yvec = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
         a.tile_clock = p->clock_on ? p->d_clock : nullptr;
         a.ntiles = ntiles; a.mode = mode;
-        return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream);
+        return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream, e0, e1);
     }
     SbSpmmArgs a{};
     a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
@@ -481,7 +495,7 @@ int plan_launch(SablePlan *p, const float *x, float *y)
     a.xmat = nullptr; a.ymat = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
     a.tile_clock = p->clock_on ? p->d_clock : nullptr;
     a.ntiles = ntiles; a.mode = mode; a.n_rhs = h.n_rhs;
-    return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream);
+    return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream, e0, e1);
 }
 
 int ensure_flush(uint32_t **buf, int64_t *n, int64_t bytes)
@@ -758,10 +772,8 @@ int sable_time(SablePlan *p, const float *x, float *y, int warmup, int reps, int
     for (int r = 0; r < reps && rc == SABLE_OK; ++r) {
         if (flush_bytes > 0) rc = flush_caches(p->d_flush, p->flush_n, p->stream, (uint32_t)(r + 7));
         if (rc) break;
-        if (hipEventRecord(e0, p->stream) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event record"); break; }
-        rc = plan_launch(p, x, y);
+        rc = plan_launch(p, x, y, e0, e1);   // events on the kernel itself
         if (rc) break;
-        if (hipEventRecord(e1, p->stream) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event record"); break; }
         if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event sync"); break; }
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);

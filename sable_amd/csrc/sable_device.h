@@ -31,6 +31,9 @@ struct SbTile {
     int32_t sp_glog;         // SpMV sparse: log2 of the lanes per row (G)
     int32_t sp_map;          // SpMV sparse: srowptr offset of the row order
                              // (nrows local rows, longest sparse row first); -1 if none
+    int32_t sp_waves;        // SpMV: wavefronts reserved for the sparse pieces (the
+                             // last ones; the rest stream the dense pieces), or 0
+                             // when every wavefront does both in turn
 };
 
 // A panel piece of a tile: the tile's rows restricted to a group of blocks

@@ -120,10 +120,15 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
 // ---------------------------------------------------------------------------
 // SpMV
 //
-// A tile's pieces are processed in staging groups: consecutive pieces whose x
-// slices (each followed by 4 zero pads) fit in the LDS buffer are staged in
-// one flattened pass (every thread issues its loads together, so the staging
-// costs one memory round trip), then processed.
+// A tile's pieces are processed in staging groups (formed by the host):
+// consecutive pieces whose x slices (each followed by 4 zero pads) fit in the
+// LDS buffer are staged in one pass through the group's slot map (x column
+// per slot), then processed.  Sparse pieces come first in a group.
+//
+// Roles: lanes tid < T' stream the dense pieces.  In a tile with both kinds
+// the last sp_waves wavefronts take the sparse pieces (their loads are
+// latency-bound and run alongside the dense stream); otherwise every
+// wavefront takes both in turn.
 //
 // Dense piece = column-major hs x ncols FP32 sub-panel, contiguous, 16-byte
 // aligned, zero-padded to a multiple of 4 floats and followed by one zero
@@ -132,25 +137,24 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
 // sees the same four row indices (4t+s) % hs and walks columns (4t+s)/hs +
 // k*M, so each lane accumulates four fixed rows in registers and every
 // wavefront load is a fully coalesced 16-byte access.  Loads are software
-// pipelined (two register batches of UNROLL float4), and the first batch of
-// a group's first dense piece is issued before the staging barrier.
+// pipelined (two register batches of UNROLL float4).
 //
 // Sparse piece = wave slices (laid out by the host plan): the tile's rows,
-// longest first, are dealt in runs of C = 64/G rows to the wavefronts, G
-// lanes per row; at step j lane q*G+g holds entries 4(jG+g) .. 4(jG+g)+3 of
-// its row (uint16 columns into the staged x slice).  A wavefront streams its
-// steps as one coalesced float4 + 4 x uint16 load per lane, accumulates in a
-// register, folds the G lanes of a row with a fixed xor butterfly and adds
-// the row sum to the row's LDS slot spy[r].  The same lane owns a row in
-// every sparse piece of the tile, so spy needs no barrier between pieces.
+// longest first, are cut into slices of C = 64/G rows, slice s going to
+// sparse wavefront s % nsw, G lanes per row; at step j lane q*G+g holds
+// entries 4(jG+g) .. 4(jG+g)+3 of its row (uint16 columns into the staged x
+// slice).  A wavefront streams a slice's steps as one coalesced float4 + 4 x
+// uint16 load per lane, accumulates in a register, folds the G lanes of a
+// row with a fixed xor butterfly and adds the row sum to the row's LDS slot
+// spy[r].  The same lane owns a row in every sparse piece of the tile, so
+// spy needs no barrier between pieces.
 //
 // Row r's result is (dense partial sums of the M column groups, in group
 // order) + (sparse sums, piece by piece): deterministic, no atomics.
 
 // Sparse steps per pipelined batch, and the waves per SIMD the SpMV kernel
-// is compiled for (its VGPR budget) given the dense unroll: two sparse steps
-// keep the kernel within 80 VGPRs at unroll 2 (6 waves) and 96 at unroll 4
-// (5 waves) without spills.
+// is compiled for (its VGPR budget) given the dense unroll: 80 VGPRs (6
+// waves) at unroll 2, 96 (5 waves) at unroll 4.
 #ifndef SB_SPUNROLL
 #define SB_SPUNROLL 2
 #endif
@@ -191,19 +195,18 @@ __device__ __forceinline__ void sb_dense_fma(const sb_f4 *v, const float *xs, co
     }
 }
 
-// Streams one dense piece whose x slice starts at xs (batch 0 preloaded in cur
-// when `preloaded`).
+// Streams one dense piece whose x slice starts at xs.
 template <int UNROLL>
 __device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int nq, int tpar, int M, int ncols,
-                                               const float *xs, const SbStreamLane &L, sb_f4 *cur,
-                                               bool preloaded, float &a0, float &a1, float &a2, float &a3)
+                                               const float *xs, const SbStreamLane &L, float &a0, float &a1,
+                                               float &a2, float &a3)
 {
     const int tid = threadIdx.x;
     if (tid >= tpar || tid >= nq) return;
     const int nmine = (nq - 1 - tid) / tpar + 1;             // float4 loads of this lane
     const int nbatch = (nmine + UNROLL - 1) / UNROLL;
-    if (!preloaded) sb_dense_load<UNROLL>(P, nq, tid, tpar, cur);
-    sb_f4 nxt[UNROLL];
+    sb_f4 cur[UNROLL], nxt[UNROLL];
+    sb_dense_load<UNROLL>(P, nq, tid, tpar, cur);
     for (int b = 0; b < nbatch; ++b) {
         // always prefetch (the batch after the last one reads the zero float4)
         sb_dense_load<UNROLL>(P, nq, tid + (b + 1) * UNROLL * tpar, tpar, nxt);
@@ -238,19 +241,20 @@ __device__ __forceinline__ void sb_stage_map(const int32_t *__restrict__ map, in
     }
 }
 
-// One wavefront's share of a sparse piece.  Loads are unconditional (the step
-// index is clamped to the last step) and software pipelined one batch ahead;
-// steps past the wavefront's count are masked to value 0 at the slice's zero
-// pad slot xs[pad].
+// One slice of a sparse piece, streamed by one wavefront: its step range
+// from the table (wave-uniform), loads unconditional (the step index is
+// clamped to the last step) and software pipelined one batch ahead; steps
+// past the slice's count are masked to value 0 at the zero pad slot xs[pad].
+// The G lanes of a row are folded with a fixed xor butterfly and the
+// row-owner lane adds the sum to spy[row].
 template <int SU>
-__device__ __forceinline__ void sb_sparse_piece(const float *__restrict__ sval, const uint16_t *__restrict__ sidx,
+__device__ __forceinline__ void sb_sparse_slice(const float *__restrict__ sval, const uint16_t *__restrict__ sidx,
                                                 const int32_t *__restrict__ tab, const int32_t *__restrict__ map,
-                                                int hs, int glog, int pad, const float *xs, float *spy)
+                                                int hs, int glog, int slice, int pad, const float *xs, float *spy)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int off = tab[2 * wave];
-    const int nst = tab[2 * wave + 1];
+    const int off = tab[2 * slice];
+    const int nst = tab[2 * slice + 1];
     if (nst <= 0) return;
     const sb_f4 *__restrict__ V = reinterpret_cast<const sb_f4 *>(sval) + (int64_t)off * 64 + lane;
     const sb_u2 *__restrict__ I = reinterpret_cast<const sb_u2 *>(sidx) + (int64_t)off * 64 + lane;
@@ -290,7 +294,7 @@ __device__ __forceinline__ void sb_sparse_piece(const float *__restrict__ sval, 
         }
     }
     for (int m = 1; m < (1 << glog); m <<= 1) acc += __shfl_xor(acc, m, 64);
-    const int i = wave * (64 >> glog) + (lane >> glog);
+    const int i = slice * (64 >> glog) + (lane >> glog);
     if ((lane & ((1 << glog) - 1)) == 0 && i < hs) spy[map[i]] += acc;
 }
 
@@ -315,8 +319,14 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         L.c3 = (p0 + 3) / hs;
     }
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    sb_f4 cur[UNROLL];
     if (tid < hs) spy[tid] = 0.f;   // before the first staging barrier; sparse row owners add after it
+    // roles: lanes tid < tpar stream the dense pieces; the last nsw
+    // wavefronts (all of them when sp_waves == 0) take the sparse slices
+    constexpr int NW = WG / 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nsw = tile.sp_waves > 0 ? tile.sp_waves : NW;
+    const int sw = wave - (NW - nsw);                     // sparse wavefront index, < 0: none
+    const int nsl = (hs + (64 >> tile.sp_glog) - 1) / (64 >> tile.sp_glog);   // sparse slices
 
     int pb = tile.piece_begin;
     while (pb < tile.piece_end) {
@@ -324,10 +334,6 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         const SbPiece first = a.pieces[pb];
         const int pe = first.grp_end;
         if (pe <= pb || first.stage_len > XCAP + 4) break;   // never built (the host and kernel choice check)
-        // issue the first dense batch before staging (it does not need x)
-        const bool pre = first.kind == SB_PIECE_DENSE && tid < tpar;
-        if (pre) sb_dense_load<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + first.data_off),
-                                       (hs * first.ncols + 3) >> 2, tid, tpar, cur);
         __syncthreads();                       // previous group done with xs
         sb_stage_map<WG>(a.xmap + first.xmap_off, first.stage_len, x, xs);
         __syncthreads();
@@ -336,11 +342,14 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
             const SbPiece pc = a.pieces[p];
             if (pc.kind == SB_PIECE_DENSE) {
                 sb_dense_piece<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off),
-                                       (hs * pc.ncols + 3) >> 2, tpar, M, pc.ncols, xs + base, L, cur, pre && p == pb,
-                                       a0, a1, a2, a3);
-            } else {
-                sb_sparse_piece<SB_SPUNROLL>(a.sval + pc.data_off, a.sidx + pc.data_off, a.srowptr + pc.rowptr_off,
-                                             a.srowptr + tile.sp_map, hs, tile.sp_glog, pc.ncols, xs + base, spy);
+                                       (hs * pc.ncols + 3) >> 2, tpar, M, pc.ncols, xs + base, L, a0, a1, a2, a3);
+            } else if (sw >= 0) {
+                // this wavefront's slices sw, sw + nsw, ...
+                for (int sl = sw; sl < nsl; sl += nsw) {
+                    sb_sparse_slice<SB_SPUNROLL>(a.sval + pc.data_off, a.sidx + pc.data_off, a.srowptr + pc.rowptr_off,
+                                                 a.srowptr + tile.sp_map, hs, tile.sp_glog, sl, pc.ncols, xs + base,
+                                                 spy);
+                }
             }
             base += pc.ncols + 4;
         }

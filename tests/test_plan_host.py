@@ -27,23 +27,24 @@ def bf16_to_f32(h):
     return (h.astype(np.uint32) << 16).view(np.float32)
 
 
-def slice_rows(t, p, srow, sval, sidx, nw):
+def slice_rows(t, p, srow, sval, sidx):
     """Decodes one SpMV wave-sliced sparse piece into {local row: (cols, vals)}
     and checks the layout: every row of the tile appears once in the row
-    order, each row's entries are a prefix of its lane chunks, and all other
-    slots are padding (value 0 at the zero pad column ncols)."""
+    order, slices of C = 64/G rows are back to back, each row's entries are a
+    prefix of its lane chunks, and all other slots are padding (value 0 at the
+    zero pad column ncols)."""
     hs, glog = int(t["nrows"]), int(t["sp_glog"])
     G, C = 1 << glog, 64 >> glog
-    assert C * nw >= hs
+    nsl = (hs + C - 1) // C
     order = srow[t["sp_map"]:t["sp_map"] + hs]
     assert sorted(order.tolist()) == list(range(hs))
-    tab = srow[p["rowptr_off"]:p["rowptr_off"] + 2 * nw].reshape(nw, 2)
+    tab = srow[p["rowptr_off"]:p["rowptr_off"] + 2 * nsl].reshape(nsl, 2)
     off = int(p["data_off"])
     assert off % 256 == 0
     assert np.array_equal(tab[:, 0], np.concatenate([[0], np.cumsum(tab[:-1, 1])]))
     ncols = int(p["ncols"])
     rows = {}
-    for w in range(nw):
+    for w in range(nsl):
         o, W = int(tab[w, 0]), int(tab[w, 1])
         blk_v = sval[off + o * 256: off + (o + W) * 256].reshape(W, 64, 4)
         blk_i = sidx[off + o * 256: off + (o + W) * 256].reshape(W, 64, 4)
@@ -86,7 +87,7 @@ def reconstruct(plan):
                 blk = dval[off:off + hs * len(cols)].reshape(len(cols), hs).T
                 M[r0:r0 + hs][:, cols] += blk
             elif p["kind"] == 1 and plan.op == "spmv":
-                for r, (lc, v) in slice_rows(t, p, srow, sval, sidx, plan.stats["wg"] // 64).items():
+                for r, (lc, v) in slice_rows(t, p, srow, sval, sidx).items():
                     assert np.all(np.diff(lc) > 0)  # ascending columns in a row
                     M[r0 + r, cols[lc]] += v
             elif p["kind"] == 1:
@@ -142,7 +143,12 @@ def test_host_plan_launch_invariants():
         assert 1 <= hs <= 256
         assert 4 * tp == M * hs and 0 < tp <= 256
         if t["sp_map"] >= 0:
-            assert 0 <= t["sp_glog"] <= 6 and (64 >> int(t["sp_glog"])) * 4 >= hs
+            assert 0 <= t["sp_glog"] <= 6
+        nw = plan.stats["wg"] // 64
+        assert 0 <= t["sp_waves"] < nw
+        if t["sp_waves"] > 0:
+            # split roles: the dense lanes fit the non-sparse wavefronts and cover the rows
+            assert tp <= 64 * (nw - t["sp_waves"]) and 64 * (nw - t["sp_waves"]) >= hs
     for p in plan.export("pieces"):
         assert p["ncols"] <= 4096
     check_staging(plan)
@@ -162,6 +168,8 @@ def check_staging(plan):
             assert p < q <= t["piece_end"]
             slots = sum(int(pieces[k]["ncols"]) + 4 for k in range(p, q))
             assert pieces[p]["stage_len"] == slots <= 4100
+            kinds = [int(pieces[k]["kind"]) for k in range(p, q)]
+            assert kinds == sorted(kinds, key=lambda k: k != 1)   # sparse pieces first
             for k in range(p + 1, q):
                 assert pieces[k]["grp_end"] == 0
                 assert pieces[k]["xmap_off"] == pieces[k - 1]["xmap_off"] + pieces[k - 1]["ncols"] + 4
