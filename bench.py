@@ -8,13 +8,16 @@ matrix resident in HBM.  By default every step starts cold: a 1 GiB buffer is
 overwritten between launches (outside the timed events), so the 256 MB
 last-level cache holds none of the matrix.
 
-Prints ONE JSON line: value = GFLOP/s (2 * nnz / time), the roofline of the
-kernel (algorithmic bytes of the executed format per launch / measured
-launch time vs 8 TB/s HBM), and the reference CPU path (the restated
-generated-C program, FP64 like the reference) timed on this host's cores on
-a bounded sample.
+Two plans are measured: the plain one (fixed density threshold T = 15) and,
+when router weights are present, the learned one (the MLP router's routes);
+the learned plan is kept only if it is within 1e-5 of the FP64 oracle and
+faster.  Prints ONE JSON line: value = GFLOP/s (2 * nnz / time) of the kept
+plan, the roofline of its kernel (algorithmic bytes of the executed format
+per launch / measured kernel time vs 8 TB/s HBM), both plans' figures, and
+the reference CPU path (the restated generated-C program, FP64 like the
+reference) timed on this host's cores on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3] [--warm]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5] [--warm]
                     [--routing plain|learned|both] [--no-cpu]
 """
 from __future__ import annotations
@@ -153,6 +156,98 @@ def bench_spmm(args, runtime, info):
     print(json.dumps(line), flush=True)
 
 
+def bench_batch(args, runtime, info):
+    """Batched workload: 256 VBR matrices (n log-uniform 2,000-64,000, P
+    uniform 50-800 partitions sized [ceil(n/5P), floor(2n/P)], 1% blocks, the
+    100k workload's density mix, seed 5), each with an SpMV and an SpMM with
+    N = 16.  Batched = learned routes (MFMA allowed for the SpMM), autotuned
+    tiles, every plan in one group (one persistent launch per operation,
+    both captured in one hipGraph).  Baseline = plain plans (fixed threshold
+    T = 15, FP32 dense path) launched one after another."""
+    from sable_amd import synth
+    from sable_amd.convert import threshold_routes
+    from sable_amd.router import Router
+    from oracle import c_oracle
+    n_rhs = 16
+    router = Router.load() if Router.available() else None
+    shapes = synth.c5_shapes(args.count)
+    plain, learned, xs, ys_p, ys_l, samples = [], [], [], [], [], []
+    flops = bytes_plain = bytes_learned = values = 0
+    t0 = time.time()
+    for i, (n, parts) in enumerate(shapes):
+        vbr, x, X = synth.workload_c5_matrix(i, n, parts, n_rhs=n_rhs)
+        values += len(vbr.val)
+        fixed = threshold_routes(vbr.val, vbr.indx, args.threshold)
+        for op, inp in (("spmv", x), ("spmm", X)):
+            k = 1 if op == "spmv" else n_rhs
+            shape = (n,) if op == "spmv" else (n, n_rhs)
+            pp = runtime.Plan.from_vbr(vbr, fixed, op=op, n_rhs=k)
+            lr = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                      vbr.bpntre, k) if router else fixed)
+            pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k)
+            plain.append(pp)
+            learned.append(pl)
+            xs.append(runtime.DeviceArray.from_numpy(inp))
+            ys_p.append(runtime.DeviceArray(shape))
+            ys_l.append(runtime.DeviceArray(shape))
+            flops += pl.stats["flops_alg"]
+            bytes_plain += pp.stats["bytes_alg"]
+            bytes_learned += pl.stats["bytes_alg"]
+        if i % 32 == 0:
+            samples.append((i, vbr, x, X))
+        del vbr
+        if (i + 1) % 16 == 0:
+            log(f"c5: {i + 1}/{len(shapes)} matrices planned, {values / 1e9:.2f}e9 stored values, "
+                f"{time.time() - t0:.0f}s")
+    flush = 0 if args.warm else 4 << 30
+    log("c5: timing the plain plans launched one by one")
+    t_plain = runtime.time_plans(plain, xs, ys_p, warmup=2, reps=args.steps, flush_bytes=flush)
+    log("c5: timing the batched group")
+    g = runtime.Group(learned, xs, ys_l)
+    t_batch = g.time(warmup=2, reps=args.steps, flush_bytes=flush)
+    log("c5: timing the plain plans as a group, the learned plans one by one")
+    g_plain = runtime.Group(plain, xs, ys_p)
+    t_batch_plain = g_plain.time(warmup=2, reps=args.steps, flush_bytes=flush)
+    g_plain.close()
+    t_learned_unbatched = runtime.time_plans(learned, xs, ys_l, warmup=1, reps=max(3, args.steps // 2),
+                                             flush_bytes=flush)
+    g.launch()
+    errs = []
+    for i, vbr, x, X in samples:
+        y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                 vbr.bpntrb, vbr.bpntre, x)
+        errs.append(c_oracle.scaled_error(ys_l[2 * i].numpy(), y64, ay)["componentwise"])
+        Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                 vbr.bpntrb, vbr.bpntre, X)
+        errs.append(c_oracle.scaled_error(ys_l[2 * i + 1].numpy().ravel(), Y64.ravel(),
+                                          AY.ravel())["componentwise"])
+    g.close()
+    ms = float(np.mean(t_batch))
+    ms_plain = float(np.mean(t_plain))
+    achieved = bytes_learned / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "VBR SpMV/SpMM GFLOP/s + % HBM/MFMA peak, learned-routed vs plain, 1 GPU",
+        "value": flops / (ms * 1e-3) / 1e9, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (SpMM dense blocks on bf16x6 MFMA where routed)", "data": "synthetic",
+        "config": {"workload": (f"{len(shapes)} VBR matrices, n log-uniform 2000-64000, P uniform 50-800, "
+                                f"1% blocks, c2 density mix, seed 5; SpMV + SpMM N=16 each"),
+                   "routing": "learned" if router else "fixed T=15 (no router weights)",
+                   "cache": "warm" if args.warm else "cold (4 GiB flush before each repetition)",
+                   "stored_values": values, "plans": len(learned)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+        "unbatched_plain_ms": ms_plain, "batched_plain_ms": float(np.mean(t_batch_plain)),
+        "unbatched_learned_ms": float(np.mean(t_learned_unbatched)),
+        "speedup_vs_unbatched_plain": ms_plain / ms,
+        "bytes_alg_learned": bytes_learned, "bytes_alg_plain": bytes_plain, "flops_alg": flops,
+        "sample_errors_vs_fp64": {"max": max(errs), "checked_products": len(errs)},
+        "device": info,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def spmv_options(args):
     """Plan options: the autotuned table's winners unless given on the command line."""
     from sable_amd.autotune import tuned_options
@@ -168,9 +263,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--count", type=int, default=256, help="c5: number of matrices")
     ap.add_argument("--warm", action="store_true", help="no cache flush between launches")
-    ap.add_argument("--routing", default="plain", choices=["plain", "learned", "both"])
+    ap.add_argument("--routing", default="both", choices=["plain", "learned", "both"],
+                    help="both: measure the learned plan beside the plain one, keep it if faster")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=None, help="default: the autotuned value")
     ap.add_argument("--tile-bytes", type=int, default=None, help="default: the autotuned value")
@@ -198,48 +295,79 @@ def main():
     elif args.workload == "c1":
         vbr, x = synth.workload_c1()
         desc = "SpMV FP32, 1000x1000 VBR, 20x20 random partitions, 40 dense blocks, seed 1"
+    elif args.workload == "c5":
+        return bench_batch(args, runtime, info)
     else:
         return bench_spmm(args, runtime, info)
     log(f"workload generated in {time.time() - t0:.1f}s: {len(vbr.val)} stored values, {len(vbr.bindx)} blocks")
-    routes = threshold_routes(vbr.val, vbr.indx, args.threshold)
     opts = spmv_options(args)
-    t1 = time.time()
-    plan = runtime.Plan.from_vbr(vbr, routes, **opts)
-    inspector_s = time.time() - t1
-    st = plan.stats
-    log(f"plan: {st['ntiles']} tiles, rtc={st['rtc_used']}, grid={st['grid']}, bytes_alg={st['bytes_alg']}, "
-        f"built in {inspector_s:.2f}s")
     xd = runtime.DeviceArray.from_numpy(x)
-    yd = runtime.DeviceArray(plan.nrows)
+    yd = runtime.DeviceArray(int(vbr.rpntr[-1]))
     flush = 0 if args.warm else FLUSH_BYTES
-    times = plan.time(xd, yd, warmup=args.warmup, reps=args.steps, flush_bytes=flush)
-    y = yd.numpy()
     y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
                                              vbr.bpntrb, vbr.bpntre, x)
-    errs = c_oracle.scaled_error(y, y64, ay)
-    ms = float(np.mean(times))
-    med = float(np.median(times))
-    gflops = st["flops_alg"] / (ms * 1e-3) / 1e9
-    achieved = st["bytes_alg"] / (ms * 1e-3) / 1e9
-    # warm numbers for reference (the MALL holds part of the matrix)
-    warm = plan.time(xd, yd, warmup=3, reps=max(5, args.steps // 2), flush_bytes=0)
+
+    def measure(routes, label):
+        t1 = time.time()
+        plan = runtime.Plan.from_vbr(vbr, routes, **opts)
+        inspector_s = time.time() - t1
+        st = plan.stats
+        log(f"{label} plan: {st['ntiles']} tiles, rtc={st['rtc_used']}, grid={st['grid']}, "
+            f"bytes_alg={st['bytes_alg']}, built in {inspector_s:.2f}s")
+        times = plan.time(xd, yd, warmup=args.warmup, reps=args.steps, flush_bytes=flush)
+        errs = c_oracle.scaled_error(yd.numpy(), y64, ay)
+        # warm numbers for reference (the MALL holds part of the matrix)
+        warm = plan.time(xd, yd, warmup=3, reps=max(5, args.steps // 2), flush_bytes=0)
+        ms = float(np.mean(times))
+        r = {"ms": ms, "median_ms": float(np.median(times)), "p10_ms": float(np.percentile(times, 10)),
+             "p90_ms": float(np.percentile(times, 90)), "warm_ms": float(np.mean(warm)),
+             "gflops": st["flops_alg"] / (ms * 1e-3) / 1e9, "gbs": st["bytes_alg"] / (ms * 1e-3) / 1e9,
+             "error_vs_fp64": errs, "bytes_alg": st["bytes_alg"], "flops_alg": st["flops_alg"],
+             "inspector_s": inspector_s,
+             "plan": {k: st[k] for k in ("ntiles", "npieces", "dense_values", "sparse_nnz", "sparse_padded",
+                                         "rowptr_entries", "xmap_entries", "nblocks_dense", "nblocks_sparse",
+                                         "rtc_used", "grid", "mode", "wg")}}
+        log(f"{label}: {ms * 1e3:.1f} us, {r['gbs']:.0f} GB/s, error {errs['componentwise']:.2e}")
+        plan.close()
+        return r
+
+    routes = threshold_routes(vbr.val, vbr.indx, args.threshold)
+    res = {"plain": measure(routes, "plain")}
+    kept = "plain"
+    if args.routing in ("learned", "both"):
+        from sable_amd.router import Router
+        if Router.available():
+            router = Router.load()
+            lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                     vbr.bpntre, 1)
+            res["learned"] = measure(lr, "learned")
+            res["learned"]["routes_changed"] = int(np.sum(lr != routes))
+            ok = res["learned"]["error_vs_fp64"]["componentwise"] <= 1e-5
+            # keep-if-faster at identical tolerance
+            if args.routing == "learned" or (ok and res["learned"]["ms"] < res["plain"]["ms"]):
+                kept = "learned"
+        else:
+            log("no router weights; learned routing skipped")
+    head = res[kept]
+    ms, gflops, achieved = head["ms"], head["gflops"], head["gbs"]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_spmv_{args.workload}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get(kept, {}).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
-        res, kind_flags, cpu_err, cores = cpu_baseline_spmv(vbr, routes, x, y64, ay)
-        mc = res["allcores"]["median_s"]
-        m1 = res["1thread"]["median_s"]
-        cpu = {"value": st["flops_alg"] / mc / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
+        cres, kind_flags, cpu_err, cores = cpu_baseline_spmv(vbr, routes, x, y64, ay)
+        mc = cres["allcores"]["median_s"]
+        m1 = cres["1thread"]["median_s"]
+        fl = head["flops_alg"]
+        cpu = {"value": fl / mc / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
                "sample": (f"whole matrix, FP64 restated generated program (CSR part + dense block calls), "
-                          f"{res['allcores']['bench']} timed passes on {cores} threads (LPT block-row chunks), "
+                          f"{cres['allcores']['bench']} timed passes on {cores} threads (LPT block-row chunks), "
                           f"median; built with {kind_flags}"),
-               "single_thread_gflops": st["flops_alg"] / m1 / 1e9,
-               "single_thread_passes": res["1thread"]["bench"],
+               "single_thread_gflops": fl / m1 / 1e9,
+               "single_thread_passes": cres["1thread"]["bench"],
                "error_vs_fp64_oracle": cpu_err,
                "gpu_speedup_vs_allcores": mc / (ms * 1e-3)}
 
@@ -248,24 +376,25 @@ def main():
         "value": gflops, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": desc, "routing": f"plain fixed threshold T={args.threshold:g}",
+        "config": {"workload": desc,
+                   "routing": (f"{kept} (learned kept only if faster at 1e-5; plain = fixed threshold "
+                               f"T={args.threshold:g})" if "learned" in res
+                               else f"plain fixed threshold T={args.threshold:g}"),
                    "cache": "warm" if args.warm else "cold (1 GiB flush between launches)",
-                   "nrows": plan.nrows, "ncols": plan.ncols, "nnz": st["nnz"],
+                   "nrows": int(vbr.rpntr[-1]), "ncols": int(vbr.cpntr[-1]), "nnz": head["flops_alg"] // 2,
                    "stored_values": int(len(vbr.val)), "blocks": int(len(vbr.bindx))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
-        "median_ms": med, "p10_ms": float(np.percentile(times, 10)), "p90_ms": float(np.percentile(times, 90)),
-        "warm_ms": float(np.mean(warm)),
-        "warm_gbs": st["bytes_alg"] / (float(np.mean(warm)) * 1e-3) / 1e9,
-        "error_vs_fp64": errs, "bytes_alg": st["bytes_alg"], "inspector_s": inspector_s,
-        "plan": dict({k: st[k] for k in ("ntiles", "npieces", "dense_values", "sparse_nnz", "rowptr_entries",
-                                         "nblocks_dense", "nblocks_sparse", "rtc_used", "grid", "mode", "wg")},
-                     options=opts),
+        "kept": kept,
+        "median_ms": head["median_ms"], "p10_ms": head["p10_ms"], "p90_ms": head["p90_ms"],
+        "warm_ms": head["warm_ms"], "error_vs_fp64": head["error_vs_fp64"], "bytes_alg": head["bytes_alg"],
+        "routings": res,
+        "learned_speedup_vs_plain": (res["plain"]["ms"] / res["learned"]["ms"]) if "learned" in res else None,
+        "options": opts,
         "device": info,
     }
     print(json.dumps(line), flush=True)
-
 
 if __name__ == "__main__":
     main()

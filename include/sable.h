@@ -76,13 +76,16 @@ typedef struct {
     int64_t tile_bytes;       /* target algorithmic bytes per tile */
     int32_t max_seg_rows;     /* upper bound on tile height (<= wg) */
     int32_t min_seg_rows;     /* lower bound on tile height when splitting */
-    int32_t mode;             /* SABLE_MODE_*; -1 = default (queue) */
+    int32_t mode;             /* SABLE_MODE_*; -1 = default (grid) */
     int32_t grid;             /* persistent grid size; 0 = 8 per CU */
     int32_t unroll;           /* SpMV dense stream unroll (1,2,4,8); 0 = 4 */
     int32_t use_rtc;          /* 1 = hipRTC per-plan specialisation (default) */
     int32_t host_only;        /* 1 = build the host plan only (no device) */
     int32_t mfma_split;       /* MFMA route: 3 = exact 3-term bf16 split, 6 passes
                                  (FP32-level, default); 2 = hi+lo, 3 passes */
+    int32_t min_tiles;        /* tiles wanted at least: while a plan has fewer, its
+                                 tile_bytes is halved (down to 16 KiB) so small
+                                 matrices still fill the device; 0 = 1024 */
 } SablePlanOpts;
 
 typedef struct {
@@ -160,15 +163,26 @@ int sable_time(SablePlan *plan, const float *x, float *y, int warmup, int reps,
 int sable_plan_enable_tile_clock(SablePlan *plan, int enable);
 int sable_tile_times(SablePlan *plan, float *ns, int32_t ntiles);
 
-/* Batching: one persistent launch over the tiles of many plans (each with
- * its own x/y), captured once into a hipGraph and replayed.  The plans must
- * share op and n_rhs. */
+/* Batching (no single reference counterpart: replaces running many generated
+ * programs, bench_launcher.py:85 / eval.py:30, one process each).  The
+ * plans' tiles (each plan with its own x/y) run in persistent launches over
+ * one LPT-ordered work list per (op, n_rhs, wg) class of members -- one
+ * launch when all members share them, e.g. SpMV then SpMM for mixed
+ * groups -- captured once into a hipGraph and replayed by
+ * sable_group_launch.  mode as in SablePlanOpts (queue by default). */
 int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x,
                        float *const *y, int mode, int grid, SableGroup **out);
 int sable_group_launch(SableGroup *g, int sync);
 int sable_group_time(SableGroup *g, int warmup, int reps, int64_t flush_bytes,
                      float *times_ms);
 int sable_group_destroy(SableGroup *g);
+/* The unbatched counterpart of a group (no reference counterpart; the
+ * baseline a group is measured against): every plan's own kernel launched
+ * back to back on the first plan's stream, each with its own x/y; times_ms[r]
+ * is the whole sequence, HIP events around it, a cache flush of flush_bytes
+ * before it. */
+int sable_time_plans(SablePlan *const *plans, int nplans, const float *const *x, float *const *y,
+                     int warmup, int reps, int64_t flush_bytes, float *times_ms);
 
 /* Router network (2-layer MLP, FP32): forward, backward and Adam kernels.
  * Weights are device pointers laid out [W1 (H x F), b1 (H), W2 (K x H), b2 (K)]. */

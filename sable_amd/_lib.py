@@ -51,7 +51,8 @@ class SablePlanOpts(C.Structure):
     _fields_ = [("op", I32), ("n_rhs", I32), ("wg", I32), ("xcap", I32), ("tile_bytes", I64),
                 ("max_seg_rows", I32), ("min_seg_rows", I32),
                 ("mode", I32), ("grid", I32), ("unroll", I32), ("use_rtc", I32),
-                ("host_only", I32), ("mfma_split", I32)]
+                ("host_only", I32), ("mfma_split", I32),
+                ("min_tiles", I32)]
 
 
 class SablePlanStats(C.Structure):
@@ -102,6 +103,7 @@ _SIGS = {
     "sable_group_launch": (I32, [P, I32]),
     "sable_group_time": (I32, [P, I32, I32, I64, F32P]),
     "sable_group_destroy": (I32, [P]),
+    "sable_time_plans": (I32, [C.POINTER(P), I32, C.POINTER(P), C.POINTER(P), I32, I32, I64, F32P]),
     "sable_mlp_forward": (I32, [P, I32, I32, I32, P, I32, P, P]),
     "sable_mlp_backward": (I32, [P, I32, I32, I32, P, I32, P, P, P]),
     "sable_mlp_mse_grad": (I32, [P, P, P, I64, C.c_float, P]),

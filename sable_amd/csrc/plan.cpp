@@ -34,12 +34,13 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     if (o.op == SABLE_OP_SPMV) o.n_rhs = 1;
     if (o.wg <= 0) o.wg = 256;
     if (o.xcap <= 0) o.xcap = 4095;
-    if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 48 * 1024 : 64 * 1024;
+    if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 128 * 1024 : 64 * 1024;
     if (o.max_seg_rows <= 0) o.max_seg_rows = o.wg;
     if (o.min_seg_rows <= 0) o.min_seg_rows = (o.op == SABLE_OP_SPMV) ? 8 : 16;
-    if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_QUEUE;
+    if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_GRID;
     if (o.unroll <= 0) o.unroll = 4;
     if (o.mfma_split != 2) o.mfma_split = 3;
+    if (o.min_tiles <= 0) o.min_tiles = 1024;
     return o;
 }
 
@@ -309,8 +310,27 @@ double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::ve
     return cost;
 }
 
+namespace {
+int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o, HostPlan &P,
+                         std::string &err);
+}
+
 int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o,
                     HostPlan &P, std::string &err)
+{
+    // Small matrices: halve the tile target until there are min_tiles tiles
+    // (or tiles are 16 KiB), so the launch still spreads over the device.
+    SablePlanOpts oo = o;
+    for (;;) {
+        const int rc = build_host_plan_once(v, route, oo, P, err);
+        if (rc || (int64_t)P.tiles.size() >= oo.min_tiles || oo.tile_bytes <= 16 * 1024) return rc;
+        oo.tile_bytes = std::max<int64_t>(16 * 1024, oo.tile_bytes / 2);
+    }
+}
+
+namespace {
+int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o, HostPlan &P,
+                         std::string &err)
 {
     int rc = validate_vbr(v, err);
     if (rc) return rc;
@@ -553,5 +573,6 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
     S.mode = o.mode;
     return SABLE_OK;
 }
+}  // namespace
 
 }  // namespace sable

@@ -19,6 +19,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <array>
 #include <vector>
 #include <sys/stat.h>
 #include <sys/time.h>
@@ -320,9 +321,10 @@ struct SablePlan {
     int32_t spec[6] = {0, 0, 0, 0, 0, 0};   // kind, wg, unroll, xcap, nrhs, rtmax
 };
 
-struct SableGroup {
-    std::vector<SablePlan *> plans;
-    hipStream_t stream = nullptr;
+// One persistent launch of a group: the members that share op, RHS width and
+// workgroup size, their descriptors and data concatenated on the device.
+struct GroupPart {
+    int32_t op = SABLE_OP_SPMV, n_rhs = 1, wg = 256, mode = SABLE_MODE_QUEUE, grid = 0, ntiles = 0;
     SbTile *d_tiles = nullptr;
     SbPiece *d_pieces = nullptr;
     SbXRange *d_xr = nullptr;
@@ -334,11 +336,15 @@ struct SableGroup {
     int32_t *d_queue = nullptr;
     const float **d_xptr = nullptr;
     float **d_yptr = nullptr;
-    int32_t ntiles = 0;
-    int32_t op = 0, n_rhs = 1, mode = SABLE_MODE_QUEUE, grid = 0;
     hipFunction_t fn = nullptr;
     const void *aot = nullptr;
     bool rtc = false;
+};
+
+struct SableGroup {
+    std::vector<SablePlan *> plans;
+    std::vector<GroupPart> parts;
+    hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t *d_flush = nullptr;
@@ -804,44 +810,49 @@ int sable_tile_times(SablePlan *p, float *ns, int32_t ntiles)
     return fail(SABLE_ERR_UNSUPPORTED, "per-tile clocks are disabled in this build");
 }
 
-// ---- groups: many plans in one persistent launch, captured in a hipGraph
+// ---- groups: many plans in persistent launches, captured in one hipGraph
 
-int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x, float *const *y,
-                       int mode, int grid, SableGroup **out)
+namespace {
+
+void free_part(GroupPart &q)
 {
-    if (!plans || nplans <= 0 || !x || !y || !out) return fail(SABLE_ERR_INVALID, "null argument");
-    *out = nullptr;
-    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
-    std::unique_ptr<SableGroup> g(new SableGroup());
-    g->op = plans[0]->host.op;
-    g->n_rhs = plans[0]->host.n_rhs;
-    g->mode = (mode < 0 || mode > 2) ? SABLE_MODE_QUEUE : mode;
-    // concatenate descriptors; data arrays are concatenated on the device
+    void *bufs[] = {q.d_tiles, q.d_pieces, q.d_xr, q.d_dval, q.d_sval, q.d_sidx, q.d_srowptr,
+                    q.d_xmap, q.d_queue, q.d_xptr, q.d_yptr};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    q = GroupPart();
+}
+
+// Builds one part from members idx (all sharing op, n_rhs and wg): their
+// descriptors rebased and concatenated in one global LPT order, their data
+// copied back to back on the device, one kernel specialisation.
+int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float *const *x, float *const *y,
+               int mode, int grid, GroupPart &q)
+{
+    SablePlan *p0 = plans[idx[0]];
+    q.op = p0->host.op;
+    q.n_rhs = p0->host.n_rhs;
+    q.wg = p0->opts.wg;
+    q.mode = (mode < 0 || mode > 2) ? SABLE_MODE_QUEUE : mode;
     std::vector<SbTile> tiles;
     std::vector<SbPiece> pieces;
     std::vector<SbXRange> xr;
     std::vector<double> cost;
     int64_t dval_n = 0, sval_n = 0, srow_n = 0, xmap_n = 0;
-    int maxcols = 1, wg = plans[0]->opts.wg;
-    bool mfma = false;
-    for (int i = 0; i < nplans; ++i) {
-        SablePlan *p = plans[i];
-        if (!p || p->host_only) return fail(SABLE_ERR_INVALID, "group member must be a device plan");
-        if (p->host.op != g->op || p->host.n_rhs != g->n_rhs || p->opts.wg != wg)
-            return fail(SABLE_ERR_INVALID, "group members must share op, n_rhs and wg");
+    for (size_t m = 0; m < idx.size(); ++m) {
+        SablePlan *p = plans[idx[m]];
         const int32_t pbase = (int32_t)pieces.size(), xbase = (int32_t)xr.size();
         for (auto pc : p->host.pieces) {
             pc.xr_begin += xbase;
             pc.xr_end += xbase;
             if (pc.kind == SB_PIECE_DENSE || pc.kind == SB_PIECE_MFMA) {
                 pc.data_off += dval_n;
-                mfma = mfma || pc.kind == SB_PIECE_MFMA;
             } else {
                 pc.data_off += sval_n;   // multiples of 256 entries keep SpMV float4 alignment
                 pc.rowptr_off += (int32_t)srow_n;
             }
             pc.xmap_off += (int32_t)xmap_n;
-            maxcols = std::max(maxcols, pc.ncols);
+            if (pc.stage_len > 0) pc.grp_end += pbase;   // first piece of a staging group
             pieces.push_back(pc);
         }
         xr.insert(xr.end(), p->host.xr.begin(), p->host.xr.end());
@@ -849,7 +860,7 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
             SbTile tt = p->host.tiles[t];
             tt.piece_begin += pbase;
             tt.piece_end += pbase;
-            tt.mat = i;
+            tt.mat = (int32_t)m;
             if (tt.sp_map >= 0) tt.sp_map += (int32_t)srow_n;
             tiles.push_back(tt);
             cost.push_back(p->host.tile_cost[t]);
@@ -859,24 +870,25 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
         srow_n += p->stats.rowptr_entries;
         xmap_n += p->stats.xmap_entries;
     }
-    if (xmap_n > INT32_MAX) return fail(SABLE_ERR_UNSUPPORTED, "group x staging map exceeds 2^31 entries");
+    if (xmap_n > INT32_MAX || srow_n > INT32_MAX || pieces.size() > (size_t)INT32_MAX)
+        return fail(SABLE_ERR_UNSUPPORTED, "group index arrays exceed 2^31 entries");
     // global LPT order
     std::vector<int32_t> order(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
     std::vector<SbTile> sorted(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
-    g->ntiles = (int32_t)sorted.size();
+    q.ntiles = (int32_t)sorted.size();
     int64_t bytes = 0;
-    CHECKRC(upload(&g->d_tiles, sorted, bytes));
-    CHECKRC(upload(&g->d_pieces, pieces, bytes));
-    CHECKRC(upload(&g->d_xr, xr, bytes));
+    CHECKRC(upload(&q.d_tiles, sorted, bytes));
+    CHECKRC(upload(&q.d_pieces, pieces, bytes));
+    CHECKRC(upload(&q.d_xr, xr, bytes));
     auto cat = [&](auto **dst, int64_t n, auto getter, size_t esz) -> int {
         *dst = nullptr;
         if (n == 0) return SABLE_OK;
         HIPOK(hipMalloc((void **)dst, n * esz));
         int64_t off = 0;
-        for (int i = 0; i < nplans; ++i) {
+        for (int i : idx) {
             auto pr = getter(plans[i]);
             if (pr.second) {
                 HIPOK(hipMemcpy((char *)*dst + off * esz, pr.first, pr.second * esz, hipMemcpyDeviceToDevice));
@@ -886,54 +898,112 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
         return SABLE_OK;
     };
     using P2 = std::pair<const void *, int64_t>;
-    CHECKRC(cat(&g->d_dval, dval_n,
+    CHECKRC(cat(&q.d_dval, dval_n,
                 [](SablePlan *p) { return P2(p->d_dval, p->stats.dense_padded + p->stats.mfma_padded); }, 4));
-    CHECKRC(cat(&g->d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_padded); }, 4));
-    CHECKRC(cat(&g->d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_padded); }, 2));
-    CHECKRC(cat(&g->d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
-    CHECKRC(cat(&g->d_xmap, xmap_n, [](SablePlan *p) { return P2(p->d_xmap, p->stats.xmap_entries); }, 4));
-    HIPOK(hipMalloc((void **)&g->d_queue, 2 * sizeof(int32_t)));
-    HIPOK(hipMemset(g->d_queue, 0, 2 * sizeof(int32_t)));
-    HIPOK(hipMalloc((void **)&g->d_xptr, nplans * sizeof(float *)));
-    HIPOK(hipMalloc((void **)&g->d_yptr, nplans * sizeof(float *)));
-    HIPOK(hipMemcpy(g->d_xptr, x, nplans * sizeof(float *), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(g->d_yptr, y, nplans * sizeof(float *), hipMemcpyHostToDevice));
-    // kernel: reuse the specialisation machinery through a throwaway host plan view
+    CHECKRC(cat(&q.d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_padded); }, 4));
+    CHECKRC(cat(&q.d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_padded); }, 2));
+    CHECKRC(cat(&q.d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
+    CHECKRC(cat(&q.d_xmap, xmap_n, [](SablePlan *p) { return P2(p->d_xmap, p->stats.xmap_entries); }, 4));
+    HIPOK(hipMalloc((void **)&q.d_queue, 2 * sizeof(int32_t)));
+    HIPOK(hipMemset(q.d_queue, 0, 2 * sizeof(int32_t)));
+    std::vector<const float *> xs;
+    std::vector<float *> ys;
+    for (int i : idx) {
+        xs.push_back(x[i]);
+        ys.push_back(y[i]);
+    }
+    HIPOK(hipMalloc((void **)&q.d_xptr, xs.size() * sizeof(float *)));
+    HIPOK(hipMalloc((void **)&q.d_yptr, ys.size() * sizeof(float *)));
+    HIPOK(hipMemcpy(q.d_xptr, xs.data(), xs.size() * sizeof(float *), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(q.d_yptr, ys.data(), ys.size() * sizeof(float *), hipMemcpyHostToDevice));
+    // kernel: the specialisation machinery over a host plan view of the part
     sable::HostPlan view;
-    view.op = g->op;
-    view.n_rhs = g->n_rhs;
+    view.op = q.op;
+    view.n_rhs = q.n_rhs;
     view.tiles = sorted;
     view.pieces = pieces;
     int32_t spec[6];
-    CHECKRC(select_kernel(view, plans[0]->opts, g->rtc, g->fn, g->aot, spec));
-    g->grid = g->mode == SABLE_MODE_GRID ? g->ntiles
-                                          : occupancy_grid(g->rtc, g->fn, g->aot, wg, std::max(1, g->ntiles), grid);
-    for (int i = 0; i < nplans; ++i) g->plans.push_back(plans[i]);
-    HIPOK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-    // capture the single persistent launch
-    HIPOK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    int rc;
-    if (g->op == SABLE_OP_SPMV) {
+    CHECKRC(select_kernel(view, p0->opts, q.rtc, q.fn, q.aot, spec));
+    q.grid = q.mode == SABLE_MODE_GRID ? q.ntiles
+                                       : occupancy_grid(q.rtc, q.fn, q.aot, q.wg, std::max(1, q.ntiles), grid);
+    return SABLE_OK;
+}
+
+int launch_part(const GroupPart &q, hipStream_t st)
+{
+    if (q.ntiles == 0) return SABLE_OK;
+    if (q.op == SABLE_OP_SPMV) {
         SbSpmvArgs a{};
-        a.tiles = g->d_tiles; a.pieces = g->d_pieces; a.xr = g->d_xr; a.dval = g->d_dval;
-        a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr; a.xmap = g->d_xmap;
-        a.xvec = g->d_xptr; a.yvec = g->d_yptr; a.queue = g->d_queue;
-        a.ntiles = g->ntiles; a.mode = g->mode;
-        rc = launch_raw(g->rtc, g->fn, g->aot, g->grid, wg, &a, g->stream);
-    } else {
-        SbSpmmArgs a{};
-        a.tiles = g->d_tiles; a.pieces = g->d_pieces; a.xr = g->d_xr; a.dval = g->d_dval;
-        a.sval = g->d_sval; a.sidx = g->d_sidx; a.srowptr = g->d_srowptr;
-        a.xmat = g->d_xptr; a.ymat = g->d_yptr; a.queue = g->d_queue;
-        a.ntiles = g->ntiles; a.mode = g->mode; a.n_rhs = g->n_rhs;
-        rc = launch_raw(g->rtc, g->fn, g->aot, g->grid, wg, &a, g->stream);
+        a.tiles = q.d_tiles; a.pieces = q.d_pieces; a.xr = q.d_xr; a.dval = q.d_dval;
+        a.sval = q.d_sval; a.sidx = q.d_sidx; a.srowptr = q.d_srowptr; a.xmap = q.d_xmap;
+        a.xvec = q.d_xptr; a.yvec = q.d_yptr; a.queue = q.d_queue;
+        a.ntiles = q.ntiles; a.mode = q.mode;
+        return launch_raw(q.rtc, q.fn, q.aot, q.grid, q.wg, &a, st);
+    }
+    SbSpmmArgs a{};
+    a.tiles = q.d_tiles; a.pieces = q.d_pieces; a.xr = q.d_xr; a.dval = q.d_dval;
+    a.sval = q.d_sval; a.sidx = q.d_sidx; a.srowptr = q.d_srowptr;
+    a.xmat = q.d_xptr; a.ymat = q.d_yptr; a.queue = q.d_queue;
+    a.ntiles = q.ntiles; a.mode = q.mode; a.n_rhs = q.n_rhs;
+    return launch_raw(q.rtc, q.fn, q.aot, q.grid, q.wg, &a, st);
+}
+
+}  // namespace
+
+int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x, float *const *y,
+                       int mode, int grid, SableGroup **out)
+{
+    if (!plans || nplans <= 0 || !x || !y || !out) return fail(SABLE_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    for (int i = 0; i < nplans; ++i)
+        if (!plans[i] || plans[i]->host_only) return fail(SABLE_ERR_INVALID, "group member must be a device plan");
+    std::unique_ptr<SableGroup> g(new SableGroup());
+    // members sharing (op, n_rhs, wg) form one part, in order of first appearance
+    std::vector<std::vector<int>> members;
+    std::vector<std::array<int, 3>> keys;
+    for (int i = 0; i < nplans; ++i) {
+        const std::array<int, 3> k = {plans[i]->host.op, plans[i]->host.n_rhs, plans[i]->opts.wg};
+        size_t j = 0;
+        while (j < keys.size() && keys[j] != k) ++j;
+        if (j == keys.size()) {
+            keys.push_back(k);
+            members.emplace_back();
+        }
+        members[j].push_back(i);
+        g->plans.push_back(plans[i]);
+    }
+    g->parts.resize(members.size());
+    for (size_t j = 0; j < members.size(); ++j) {
+        const int rc = build_part(plans, members[j], x, y, mode, grid, g->parts[j]);
+        if (rc) {
+            for (auto &q : g->parts) free_part(q);
+            return rc;
+        }
+    }
+    HIPOK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    // capture every part's persistent launch, one after the other
+    HIPOK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = SABLE_OK;
+    for (auto &q : g->parts) {
+        rc = launch_part(q, g->stream);
+        if (rc) break;
     }
     hipGraph_t graph = nullptr;
     hipError_t ce = hipStreamEndCapture(g->stream, &graph);
-    if (rc) return rc;
-    if (ce != hipSuccess) return fail(SABLE_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-    g->graph = graph;
-    HIPOK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    if (rc == SABLE_OK && ce != hipSuccess)
+        rc = fail(SABLE_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    if (rc == SABLE_OK) {
+        g->graph = graph;
+        if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = fail(SABLE_ERR_HIP, "hipGraphInstantiate failed");
+    } else if (graph) {
+        hipGraphDestroy(graph);
+    }
+    if (rc) {
+        sable_group_destroy(g.release());
+        return rc;
+    }
     *out = g.release();
     return SABLE_OK;
 }
@@ -971,16 +1041,60 @@ int sable_group_time(SableGroup *g, int warmup, int reps, int64_t flush_bytes, f
     return SABLE_OK;
 }
 
+int sable_time_plans(SablePlan *const *plans, int nplans, const float *const *x, float *const *y, int warmup,
+                     int reps, int64_t flush_bytes, float *times_ms)
+{
+    if (!plans || nplans <= 0 || !x || !y) return fail(SABLE_ERR_INVALID, "null argument");
+    if (reps < 0 || warmup < 0 || (reps > 0 && !times_ms)) return fail(SABLE_ERR_INVALID, "bad reps");
+    for (int i = 0; i < nplans; ++i)
+        if (!plans[i] || plans[i]->host_only) return fail(SABLE_ERR_INVALID, "every plan must be a device plan");
+    SablePlan *p0 = plans[0];
+    hipStream_t st = p0->stream;
+    // every launch goes to the first plan's stream, back to back
+    auto run_all = [&]() -> int {
+        for (int i = 0; i < nplans; ++i) {
+            hipStream_t keep = plans[i]->stream;
+            plans[i]->stream = st;
+            const int rc = plan_launch(plans[i], x[i], y[i]);
+            plans[i]->stream = keep;
+            if (rc) return rc;
+        }
+        return SABLE_OK;
+    };
+    if (flush_bytes > 0) CHECKRC(ensure_flush(&p0->d_flush, &p0->flush_n, flush_bytes));
+    for (int i = 0; i < warmup; ++i) {
+        if (flush_bytes > 0) CHECKRC(flush_caches(p0->d_flush, p0->flush_n, st, (uint32_t)i));
+        CHECKRC(run_all());
+    }
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    int rc = SABLE_OK;
+    for (int r = 0; r < reps && rc == SABLE_OK; ++r) {
+        if (flush_bytes > 0) rc = flush_caches(p0->d_flush, p0->flush_n, st, (uint32_t)(r + 11));
+        if (rc) break;
+        hipEventRecord(e0, st);
+        rc = run_all();
+        if (rc) break;
+        hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(SABLE_ERR_HIP, "event sync"); break; }
+        hipEventElapsedTime(&times_ms[r], e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    HIPOK(hipStreamSynchronize(st));
+    return SABLE_OK;
+}
+
 int sable_group_destroy(SableGroup *g)
 {
     if (!g) return SABLE_OK;
     if (g->stream) hipStreamSynchronize(g->stream);
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
-    void *bufs[] = {g->d_tiles, g->d_pieces, g->d_xr, g->d_dval, g->d_sval, g->d_sidx, g->d_srowptr,
-                    g->d_xmap, g->d_queue, g->d_xptr, g->d_yptr, g->d_flush};
-    for (void *b : bufs)
-        if (b) hipFree(b);
+    for (auto &q : g->parts) free_part(q);
+    if (g->d_flush) hipFree(g->d_flush);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
     return SABLE_OK;

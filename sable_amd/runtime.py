@@ -146,14 +146,27 @@ class Plan:
 
     ``routes[k]`` (per stored block, ordinal order) is ROUTE_DENSE,
     ROUTE_SPARSE or ROUTE_DENSE_MFMA; None means all dense.
+
+    Shape options left as None take the autotuned values of ``tuning.json``
+    for the operation when ``tuned`` is true, else the library defaults.
     """
 
     def __init__(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, routes=None, *,
-                 op="spmv", n_rhs=1, host_only=False, wg=256, xcap=4095, tile_bytes=0,
-                 max_seg_rows=0, min_seg_rows=0, mode=_lib.MODE_QUEUE,
-                 grid=0, unroll=4, use_rtc=True, mfma_split=3):
+                 op="spmv", n_rhs=1, host_only=False, wg=None, xcap=4095, tile_bytes=None,
+                 max_seg_rows=0, min_seg_rows=0, mode=None,
+                 grid=0, unroll=None, use_rtc=True, mfma_split=3, min_tiles=0, tuned=True):
         if not host_only:
             init()
+        given = {"wg": wg, "tile_bytes": tile_bytes, "mode": mode, "unroll": unroll}
+        if tuned:
+            from .autotune import tuned_options
+            for k, v in tuned_options(op, n_rhs).items():
+                if k in given and given[k] is None:
+                    given[k] = v
+        wg = given["wg"] or 256
+        tile_bytes = given["tile_bytes"] or 0
+        mode = _lib.MODE_GRID if given["mode"] is None else given["mode"]
+        unroll = given["unroll"] or 0
         s, keep = _vbr_struct(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre)
         o = _lib.SablePlanOpts()
         o.op = _lib.OP_SPMM if op == "spmm" else _lib.OP_SPMV
@@ -163,6 +176,7 @@ class Plan:
         o.mode, o.grid, o.unroll = mode, grid, unroll
         o.use_rtc, o.host_only = int(bool(use_rtc)), int(bool(host_only))
         o.mfma_split = int(mfma_split)
+        o.min_tiles = int(min_tiles)
         r = None
         if routes is not None:
             r = np.ascontiguousarray(routes, dtype=np.uint8)
@@ -286,17 +300,39 @@ class Plan:
         return out
 
 
+def _dev_ptrs(plans, xs, ys):
+    """Device pointers of every plan's x and y (DeviceArrays only: these
+    launches have no host staging)."""
+    n = len(plans)
+    if len(xs) != n or len(ys) != n:
+        raise ValueError("one x and one y per plan")
+    xp, yp = [], []
+    for p, x, y in zip(plans, xs, ys):
+        if not isinstance(x, DeviceArray) or not isinstance(y, DeviceArray):
+            raise TypeError("batched launches take DeviceArrays")
+        xp.append(_ptr(x, p.ncols * p.n_rhs)[0])
+        yp.append(_ptr(y, p.nrows * p.n_rhs)[0])
+    return (_lib.P * n)(*[p._h for p in plans]), (_lib.P * n)(*xp), (_lib.P * n)(*yp)
+
+
+def time_plans(plans, xs, ys, warmup=3, reps=10, flush_bytes=0):
+    """Launches every plan's own kernel back to back (the unbatched way to
+    run many matrices) and times the whole sequence; ms per repetition."""
+    init()
+    n = len(plans)
+    hs, xp, yp = _dev_ptrs(plans, xs, ys)
+    out = np.zeros(reps, dtype=np.float32)
+    check(lib.sable_time_plans(hs, n, xp, yp, warmup, reps, int(flush_bytes), out.ctypes.data_as(_lib.F32P)))
+    return out
+
+
 class Group:
     """Many plans executed by one persistent launch replayed from a hipGraph."""
 
     def __init__(self, plans, xs, ys, mode=_lib.MODE_QUEUE, grid=0):
         init()
         n = len(plans)
-        if len(xs) != n or len(ys) != n:
-            raise ValueError("one x and one y per plan")
-        hs = (_lib.P * n)(*[p._h for p in plans])
-        xp = (_lib.P * n)(*[_ptr(x, p.ncols * p.n_rhs)[0] for p, x in zip(plans, xs)])
-        yp = (_lib.P * n)(*[_ptr(y, p.nrows * p.n_rhs)[0] for p, y in zip(plans, ys)])
+        hs, xp, yp = _dev_ptrs(plans, xs, ys)
         g = _lib.P()
         check(lib.sable_group_create(hs, n, xp, yp, mode, grid, C.byref(g)))
         self._g = g

@@ -194,6 +194,9 @@ WORKLOADS = {
                    "20-200, 1% blocks, half density 0.8-1.0 / half 0.01-0.2, N(0,1)"),
     "c3": Workload("c3", 3, "50000x50000 VBR, 500x500 partitions sized 32-192, "
                    "2% blocks, 70% density 0.9-1.0 / 30% 0.02-0.1, N(0,1), N=128"),
+    "c5": Workload("c5", 5, "256 VBR matrices, n log-uniform 2000-64000, P uniform 50-800 "
+                   "partitions sized [ceil(n/5P), floor(2n/P)], 1% blocks, density mix of c2, "
+                   "SpMV and SpMM with N=16"),
 }
 
 
@@ -216,6 +219,35 @@ def workload_c3(seed=3, n=50_000, parts=500, n_rhs=128):
     g = _rngs(seed + 1000)
     X = g["x"].uniform(-1.0, 1.0, (n, n_rhs)).astype(np.float32)
     return vbr, X, cls
+
+
+def c5_shapes(count=256, seed=5, n_range=(2000, 64000), p_range=(50, 800)):
+    """(n, P) of the batched workload's matrices: n log-uniform integer in
+    n_range, P uniform integer in p_range (at most n)."""
+    g = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, 0])))
+    lo, hi = np.log(n_range[0]), np.log(n_range[1] + 1)
+    ns = np.floor(np.exp(g.uniform(lo, hi, count))).astype(np.int64)
+    ns = np.clip(ns, n_range[0], n_range[1])
+    ps = g.integers(p_range[0], p_range[1] + 1, count)
+    return [(int(n), int(min(p, n))) for n, p in zip(ns, ps)]
+
+
+def workload_c5_matrix(i, n, parts, seed=5, n_rhs=16):
+    """Matrix i of the batched workload: partition sizes in [ceil(n/5P),
+    floor(2n/P)], 1% of blocks, the 100k workload's density mix; x U[-1,1]
+    and an n x n_rhs row-major X U[-1,1]."""
+    lo = max(1, -(-n // (5 * parts)))
+    hi = max(lo, (2 * n) // parts)
+    vbr, x, _ = mixed_vbr(n, parts, lo, hi, 0.01, 0.5, (0.8, 1.0), (0.01, 0.2), seed * 100_003 + i)
+    g = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, 1, i])))
+    X = g.uniform(-1.0, 1.0, (n, n_rhs)).astype(np.float32)
+    return vbr, x, X
+
+
+def workload_c5(count=256, seed=5, n_rhs=16, **kw):
+    """Yields (vbr, x, X) for the batched workload's matrices, one at a time."""
+    for i, (n, p) in enumerate(c5_shapes(count, seed, **kw)):
+        yield workload_c5_matrix(i, n, p, seed, n_rhs)
 
 
 def workload_like_c2(n, seed, frac_blocks=0.01):

@@ -104,8 +104,8 @@ def test_rtc_and_aot_agree(rt):
     vbr, x, _ = synth.workload_like_c2(5000, seed=8, frac_blocks=0.03)
     routes = threshold_routes(vbr.val, vbr.indx, 15)
     print("case: rtc vs aot", flush=True)
-    a = rt.Plan.from_vbr(vbr, routes, use_rtc=False)
-    b = rt.Plan.from_vbr(vbr, routes, use_rtc=True)
+    a = rt.Plan.from_vbr(vbr, routes, use_rtc=False, wg=256, unroll=4)   # the generic AOT kernel's shape
+    b = rt.Plan.from_vbr(vbr, routes, use_rtc=True, wg=256, unroll=4)
     assert b.stats["rtc_used"] == 1 and a.stats["rtc_used"] == 0
     assert np.array_equal(a.spmv(x), b.spmv(x))
 

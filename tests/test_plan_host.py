@@ -140,8 +140,9 @@ def test_host_plan_launch_invariants():
     plan = Plan.from_vbr(vbr, threshold_routes(vbr.val, vbr.indx, 15), host_only=True, tile_bytes=2048)
     for t in plan.export("tiles"):
         hs, tp, M = int(t["nrows"]), int(t["dense_tpar"]), int(t["dense_mcols"])
-        assert 1 <= hs <= 256
-        assert 4 * tp == M * hs and 0 < tp <= 256
+        wg = plan.stats["wg"]
+        assert 1 <= hs <= wg
+        assert 4 * tp == M * hs and 0 < tp <= wg
         if t["sp_map"] >= 0:
             assert 0 <= t["sp_glog"] <= 6
         nw = plan.stats["wg"] // 64
