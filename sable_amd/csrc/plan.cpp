@@ -334,7 +334,11 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
 {
     int rc = validate_vbr(v, err);
     if (rc) return rc;
-    if (o.wg != 64 && o.wg != 128 && o.wg != 256 && o.wg != 512) { err = "wg must be 64, 128, 256 or 512"; return SABLE_ERR_INVALID; }
+    if (o.op == SABLE_OP_SPMM && o.wg != 64 && o.wg != 128 && o.wg != 256 && o.wg != 512) {
+        err = "SpMM needs wg 64, 128, 256 or 512";
+        return SABLE_ERR_INVALID;
+    }
+    if (o.wg < 64 || o.wg > 1024 || o.wg % 64) { err = "wg must be a multiple of 64 in [64, 1024]"; return SABLE_ERR_INVALID; }
     if (o.xcap < 16 || o.xcap > 4095) { err = "xcap must be in [16, 4095]"; return SABLE_ERR_INVALID; }
     const bool spmm = o.op == SABLE_OP_SPMM;
     if (spmm && (o.n_rhs < 4 || o.n_rhs % 4 != 0 || o.n_rhs / 4 > o.wg)) {
@@ -441,12 +445,14 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             }
             // SpMV with dense and sparse pieces: the last sp_waves wavefronts
             // take the (latency-bound) sparse pieces while the others stream
-            // the dense ones; the dense stream needs at least hs lanes.
+            // the dense ones, in proportion to the pieces' bytes (at least
+            // one each; measured best against heavier sparse weightings);
+            // the dense stream needs at least hs lanes.
             const int nw = o.wg / 64;
             t.sp_waves = 0;
             if (!spmm && !sp.empty() && !dgroups.empty() && nw >= 2) {
                 const double sb = 6.0 * (double)sp_nnz, db = 4.0 * hs * (double)wd;
-                int k = (int)std::lround(2.0 * nw * sb / (sb + db));
+                int k = (int)std::lround(nw * sb / (sb + db));
                 k = std::max(1, std::min(k, nw - 1));
                 while (k > 0 && 64 * (nw - k) < hs) --k;
                 t.sp_waves = k;
