@@ -91,6 +91,7 @@ typedef struct {
 typedef struct {
     int32_t nrows, ncols;
     int32_t ntiles, npieces, nxranges;
+    int32_t ntiles_mfma;      /* SpMM: MFMA tiles, run first (the first ntiles_mfma tiles) */
     int32_t n_rhs;
     int64_t nblocks, nblocks_dense, nblocks_sparse, nblocks_mfma;
     int64_t nnz;              /* stored values != 0 (flops = 2 * nnz * N) */

@@ -57,7 +57,7 @@ class SablePlanOpts(C.Structure):
 
 class SablePlanStats(C.Structure):
     _fields_ = [("nrows", I32), ("ncols", I32), ("ntiles", I32), ("npieces", I32),
-                ("nxranges", I32), ("n_rhs", I32), ("nblocks", I64), ("nblocks_dense", I64),
+                ("nxranges", I32), ("ntiles_mfma", I32), ("n_rhs", I32), ("nblocks", I64), ("nblocks_dense", I64),
                 ("nblocks_sparse", I64), ("nblocks_mfma", I64), ("nnz", I64),
                 ("dense_values", I64), ("dense_padded", I64), ("mfma_values", I64),
                 ("mfma_padded", I64), ("sparse_nnz", I64), ("sparse_padded", I64),

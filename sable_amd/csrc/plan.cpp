@@ -361,8 +361,12 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     // SpMM kernel needs hs <= (wg / (N/4)) * 8 register rows.
     int max_rows = std::min(o.max_seg_rows, o.wg);
     // SpMM: the FP32 kernel holds at most 8 rows per thread group
-    // ((wg / (N/4)) * 8 rows) and the MFMA kernel at most 64 rows per tile.
+    // ((wg / (N/4)) * 8 rows, and 64 rows); MFMA tiles are cut separately:
+    // multiples of 16 rows, at most mfma_max_rows.
     if (spmm) max_rows = std::min({max_rows, (o.wg / (o.n_rhs / 4)) * 8, 64});
+    const int mfma_rows = spmm && o.n_rhs % 16 == 0
+                              ? std::max(16, std::min(SB_MFMA_MAX_ROWS(o.n_rhs, o.wg), (o.max_seg_rows / 16) * 16))
+                              : 16;
     max_rows = std::max(max_rows, 1);
     const int min_rows = std::max(1, std::min(o.min_seg_rows, max_rows));
 
@@ -409,33 +413,21 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
         for (auto &g : mgroups) wm += group_width(g);
         const double work = 4.0 * h * (double)(wd + wm) + 6.0 * sp_total + 4.0 * (h + 1) * sgroups.size();
 
-        // row segmentation
-        int64_t nseg = (h + max_rows - 1) / max_rows;
-        const int64_t want = (int64_t)std::ceil(work / (double)o.tile_bytes);
-        nseg = std::max<int64_t>(nseg, std::min<int64_t>(want, (h + min_rows - 1) / min_rows));
-        nseg = std::max<int64_t>(nseg, 1);
-        std::vector<int32_t> seg_start;
-        if (spmm && !mgroups.empty()) {
-            // MFMA tiles: heights in multiples of 16 rows where possible
-            const int64_t m16 = (h + 15) / 16;
-            nseg = std::min<int64_t>(nseg, m16);
-            for (int64_t s = 0; s < nseg; ++s) seg_start.push_back((int32_t)(16 * ((m16 * s) / nseg)));
-        } else {
-            for (int64_t s = 0; s < nseg; ++s) seg_start.push_back((int32_t)((h * s) / nseg));
-        }
-        seg_start.push_back(h);
-
-        for (size_t s = 0; s + 1 < seg_start.size(); ++s) {
-            const int32_t s0 = seg_start[s], hs = seg_start[s + 1] - s0;
-            if (hs <= 0) continue;
+        // Emits the tile of rows [s0, s0+hs) of this block row with the
+        // pieces of the selected routes (vector routes: dense FP32 and
+        // sparse; MFMA).  y_accum: the tile adds to Y rows an earlier MFMA
+        // tile of the same rows wrote.
+        const std::vector<std::vector<ColSeg>> none;
+        auto emit = [&](int32_t s0, int32_t hs, bool use_vec, bool use_mfma, int32_t y_accum) {
             SbTile t{};
             t.row0 = r0 + s0;
             t.nrows = hs;
             t.mat = 0;
+            t.y_accum = y_accum;
             // sparse rows of the segment, per sparse column group
             std::vector<SparseRows> sp;
             int64_t sp_nnz = 0;
-            for (size_t gi = 0; gi < sgroups.size(); ++gi) {
+            for (size_t gi = 0; use_vec && gi < sgroups.size(); ++gi) {
                 SparseRows sr;
                 if (collect_sparse_rows(v, sgroups[gi], h, s0, hs, sr)) {
                     sr.group = (int32_t)gi;
@@ -450,7 +442,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             // the dense stream needs at least hs lanes.
             const int nw = o.wg / 64;
             t.sp_waves = 0;
-            if (!spmm && !sp.empty() && !dgroups.empty() && nw >= 2) {
+            if (!spmm && !sp.empty() && use_vec && !dgroups.empty() && nw >= 2) {
                 const double sb = 6.0 * (double)sp_nnz, db = 4.0 * hs * (double)wd;
                 int k = (int)std::lround(nw * sb / (sb + db));
                 k = std::max(1, std::min(k, nw - 1));
@@ -462,7 +454,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             t.piece_begin = (int32_t)P.pieces.size();
             double cost = 64.0;
             // dense FP32 pieces
-            for (auto &g : dgroups) {
+            for (auto &g : (use_vec ? dgroups : none)) {
                 SbPiece pc{};
                 pc.kind = SB_PIECE_DENSE;
                 pc.ncols = group_width(g);
@@ -483,7 +475,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                 cost += 4.0 * hs * pc.ncols;
             }
             // MFMA pieces (SpMM): FP32 in A-operand register order
-            for (auto &g : mgroups) {
+            for (auto &g : (use_mfma ? mgroups : none)) {
                 SbPiece pc{};
                 pc.kind = SB_PIECE_MFMA;
                 pc.ncols = group_width(g);
@@ -546,13 +538,55 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
             P.tiles.push_back(t);
             P.tile_cost.push_back(cost);
+            P.tile_phase.push_back(use_mfma && !use_vec ? 0 : 1);
+        };
+        // segment starts: nseg segments, optionally at multiples of 16 rows
+        auto cut = [&](int64_t nseg, bool by16) {
+            std::vector<int32_t> st;
+            if (by16) {
+                const int64_t m16 = (h + 15) / 16;
+                nseg = std::max<int64_t>(1, std::min<int64_t>(nseg, m16));
+                for (int64_t s = 0; s < nseg; ++s) st.push_back((int32_t)(16 * ((m16 * s) / nseg)));
+            } else {
+                nseg = std::max<int64_t>(1, std::min<int64_t>(nseg, h));
+                for (int64_t s = 0; s < nseg; ++s) st.push_back((int32_t)((h * s) / nseg));
+            }
+            st.push_back(h);
+            return st;
+        };
+        auto vec_segments = [&](double work_v) {
+            int64_t nseg = (h + max_rows - 1) / max_rows;
+            const int64_t want = (int64_t)std::ceil(work_v / (double)o.tile_bytes);
+            return cut(std::max<int64_t>(nseg, std::min<int64_t>(want, (h + min_rows - 1) / min_rows)), false);
+        };
+        if (spmm && !mgroups.empty()) {
+            // MFMA tiles (first launch): tall, multiples of 16 rows, only the
+            // MFMA pieces, writing Y; then tiles of the other pieces (second
+            // launch) adding to Y
+            int64_t nseg = (h + mfma_rows - 1) / mfma_rows;
+            const int64_t want = (int64_t)std::ceil(4.0 * h * (double)wm / (double)o.tile_bytes);
+            const auto st = cut(std::max(nseg, want), true);
+            for (size_t s = 0; s + 1 < st.size(); ++s)
+                if (st[s + 1] > st[s]) emit(st[s], st[s + 1] - st[s], false, true, 0);
+            if (!dgroups.empty() || !sgroups.empty()) {
+                const auto sv = vec_segments(4.0 * h * (double)wd + 6.0 * sp_total + 4.0 * (h + 1) * sgroups.size());
+                for (size_t s = 0; s + 1 < sv.size(); ++s)
+                    if (sv[s + 1] > sv[s]) emit(sv[s], sv[s + 1] - sv[s], true, false, 1);
+            }
+        } else {
+            const auto sv = vec_segments(work);
+            for (size_t s = 0; s + 1 < sv.size(); ++s)
+                if (sv[s + 1] > sv[s]) emit(sv[s], sv[s + 1] - sv[s], true, false, 0);
         }
     }
     // largest tiles first (LPT order for the persistent queue)
     std::vector<int32_t> order(P.tiles.size());
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return P.tile_cost[x] > P.tile_cost[y]; });
+    // MFMA tiles (first launch) before the others; largest first in each
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        if (P.tile_phase[x] != P.tile_phase[y]) return P.tile_phase[x] < P.tile_phase[y];
+        return P.tile_cost[x] > P.tile_cost[y];
+    });
     std::vector<SbTile> tiles(P.tiles.size());
     std::vector<double> cost(P.tiles.size());
     for (size_t i = 0; i < order.size(); ++i) {
@@ -561,6 +595,9 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     }
     P.tiles.swap(tiles);
     P.tile_cost.swap(cost);
+    S.ntiles_mfma = 0;
+    for (int32_t ph : P.tile_phase) S.ntiles_mfma += (ph == 0);
+    std::sort(P.tile_phase.begin(), P.tile_phase.end());
 
     S.ntiles = (int32_t)P.tiles.size();
     S.npieces = (int32_t)P.pieces.size();

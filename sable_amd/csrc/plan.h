@@ -24,6 +24,7 @@ struct HostPlan {
     std::vector<int32_t> srowptr;
     std::vector<int32_t> xmap;    // SpMV: x column per staged slot (-1 = zero pad)
     std::vector<double> tile_cost;   // estimated bytes per tile (ordering)
+    std::vector<int32_t> tile_phase; // 0: SpMM MFMA tile (first launch), 1: the rest
     SablePlanStats stats{};
 };
 

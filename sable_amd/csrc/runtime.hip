@@ -284,6 +284,19 @@ int pow2_at_least(int v)
 // ---------------------------------------------------------------------------
 // plan object
 
+// One launch of a plan or group part: a contiguous range of its tiles and
+// the kernel that runs them.  SpMV plans and SpMM plans without MFMA pieces
+// have one phase; SpMM plans with MFMA pieces run their MFMA tiles first and
+// then the tiles of the other pieces, which add to Y.
+struct Phase {
+    int32_t t0 = 0, t1 = 0;
+    bool rtc = false;
+    hipFunction_t fn = nullptr;
+    const void *aot = nullptr;
+    int grid = 0;
+    int32_t spec[6] = {0, 0, 0, 0, 0, 0};   // kind, wg, unroll, xcap, nrhs, rtmax
+};
+
 struct SablePlan {
     sable::HostPlan host;
     bool host_only = false;
@@ -313,12 +326,8 @@ struct SablePlan {
     uint32_t *d_flush = nullptr;
     int64_t flush_n = 0;
     // kernel
-    bool rtc = false;
-    hipFunction_t fn = nullptr;
-    const void *aot = nullptr;
-    int grid = 0;
+    std::vector<Phase> phases;
     int64_t device_bytes = 0;
-    int32_t spec[6] = {0, 0, 0, 0, 0, 0};   // kind, wg, unroll, xcap, nrhs, rtmax
 };
 
 // One persistent launch of a group: the members that share op, RHS width and
@@ -336,9 +345,7 @@ struct GroupPart {
     int32_t *d_queue = nullptr;
     const float **d_xptr = nullptr;
     float **d_yptr = nullptr;
-    hipFunction_t fn = nullptr;
-    const void *aot = nullptr;
-    bool rtc = false;
+    std::vector<Phase> phases;
 };
 
 struct SableGroup {
@@ -377,68 +384,85 @@ int max_piece_cols(const sable::HostPlan &h)
     return m;
 }
 
-// x buffer (slots beyond the 4 trailing pads) the largest SpMV staging group needs.
-int max_group_staging(const sable::HostPlan &h)
+// chooses the kernel of one phase (tiles [t0, t1) of the host plan view):
+// an RTC specialisation sized for those tiles, or the generic AOT kernel
+int select_phase(const sable::HostPlan &h, const SablePlanOpts &o, Phase &ph)
 {
-    int m = 1;
-    for (auto &p : h.pieces) m = std::max(m, p.stage_len - 4);
-    return m;
-}
-
-bool has_kind(const sable::HostPlan &h, int kind)
-{
-    for (auto &p : h.pieces)
-        if (p.kind == kind) return true;
-    return false;
-}
-
-// chooses the kernel for a plan (RTC specialisation or AOT generic)
-int select_kernel(const sable::HostPlan &h, const SablePlanOpts &o, bool &rtc, hipFunction_t &fn,
-                  const void *&aot, int32_t spec_out[6])
-{
-    rtc = false;
-    fn = nullptr;
+    ph.rtc = false;
+    ph.fn = nullptr;
     Spec s;
     s.wg = o.wg;
     s.unroll = o.unroll;
     const bool spmm = h.op == SABLE_OP_SPMM;
+    bool mfma = false;
+    int maxcols = 1, maxstage = 1, rows = 1;
+    for (int32_t t = ph.t0; t < ph.t1; ++t) {
+        const SbTile &tl = h.tiles[t];
+        rows = std::max(rows, tl.nrows);
+        for (int32_t k = tl.piece_begin; k < tl.piece_end; ++k) {
+            const SbPiece &pc = h.pieces[k];
+            maxcols = std::max(maxcols, pc.ncols);
+            maxstage = std::max(maxstage, pc.stage_len - 4);
+            mfma = mfma || pc.kind == SB_PIECE_MFMA;
+        }
+    }
     // SpMV: room for the largest staging group the host formed (at most
     // SB_STAGE_CAP + 4 slots, so the buffer never exceeds 4096 + 4)
-    s.xcap = spmm ? std::min(4096, pow2_at_least(max_piece_cols(h) + 1))
-                  : std::min(sable::SB_STAGE_CAP, std::max(16, pow2_at_least(max_group_staging(h))));
-    const bool mfma = spmm && has_kind(h, SB_PIECE_MFMA);
+    s.xcap = spmm ? std::min(4096, pow2_at_least(maxcols + 1))
+                  : std::min(sable::SB_STAGE_CAP, std::max(16, pow2_at_least(maxstage)));
     s.split = o.mfma_split == 2 ? 2 : 3;
     if (spmm) {
         s.nrhs = h.n_rhs;
-        int rows = 1;
-        for (auto &t : h.tiles) rows = std::max(rows, t.nrows);
         const int ngroups = std::max(1, o.wg / (h.n_rhs / 4));
-        s.rtmax = std::max(1, (rows + ngroups - 1) / ngroups);
+        s.rtmax = mfma ? 8 : std::max(1, (rows + ngroups - 1) / ngroups);
     }
     if (spmm)
-        aot = mfma ? (s.split == 2 ? (const void *)sb_spmm_mfma2_aot : (const void *)sb_spmm_mfma_aot)
-                   : (const void *)sb_spmm_f32_aot;
-    else aot = (const void *)sb_spmv_aot;
-    spec_out[0] = spmm ? (mfma ? (s.split == 2 ? 3 : 2) : 1) : 0;
-    spec_out[1] = s.wg;
-    spec_out[2] = s.unroll;
-    spec_out[3] = s.xcap;
-    spec_out[4] = s.nrhs;
-    spec_out[5] = s.rtmax;
+        ph.aot = mfma ? (s.split == 2 ? (const void *)sb_spmm_mfma2_aot : (const void *)sb_spmm_mfma_aot)
+                      : (const void *)sb_spmm_f32_aot;
+    else ph.aot = (const void *)sb_spmv_aot;
+    ph.spec[0] = spmm ? (mfma ? (s.split == 2 ? 3 : 2) : 1) : 0;
+    ph.spec[1] = s.wg;
+    ph.spec[2] = s.unroll;
+    ph.spec[3] = s.xcap;
+    ph.spec[4] = s.nrhs;
+    ph.spec[5] = s.rtmax;
     if (!o.use_rtc) {
         if (o.wg != 256 || (!spmm && o.unroll != 4)) return fail(SABLE_ERR_UNSUPPORTED, "AOT kernels exist only for wg=256 / unroll=4");
         return SABLE_OK;
     }
     const char *stub = spmm ? (mfma ? kStubSpmmMfma : kStubSpmmF32) : kStubSpmv;
     const char *entry = spmm ? (mfma ? "sb_spmm_mfma_rtc" : "sb_spmm_f32_rtc") : "sb_spmv_rtc";
-    int rc = rtc_get(spec_defines(s), stub, entry, &fn);
+    int rc = rtc_get(spec_defines(s), stub, entry, &ph.fn);
     if (rc == SABLE_OK) {
-        rtc = true;
+        ph.rtc = true;
         return SABLE_OK;
     }
     // fall back to the generic kernel; keep the message for the caller
     if (o.wg != 256 || (!spmm && o.unroll != 4)) return rc;
     fprintf(stderr, "sable: %s; using the generic kernel\n", g_err.c_str());
+    return SABLE_OK;
+}
+
+// The phases of a plan view: [0, ntiles_mfma) on the MFMA kernel, the rest
+// on the FP32 (SpMM) or SpMV kernel.
+int select_phases(const sable::HostPlan &h, const SablePlanOpts &o, int32_t ntiles_mfma,
+                  std::vector<Phase> &phases)
+{
+    phases.clear();
+    const int32_t n = (int32_t)h.tiles.size();
+    if (ntiles_mfma > 0) {
+        Phase a;
+        a.t0 = 0;
+        a.t1 = ntiles_mfma;
+        phases.push_back(a);
+    }
+    if (n > ntiles_mfma || n == 0) {
+        Phase b;
+        b.t0 = ntiles_mfma;
+        b.t1 = n;
+        phases.push_back(b);
+    }
+    for (auto &ph : phases) CHECKRC(select_phase(h, o, ph));
     return SABLE_OK;
 }
 
@@ -477,31 +501,48 @@ int launch_raw(bool rtc, hipFunction_t fn, const void *aot, int grid, int wg, vo
     return SABLE_OK;
 }
 
-int plan_launch(SablePlan *p, const float *x, float *y, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+// Launches one phase; events (when given) time this launch alone.
+int launch_phase(int op, int n_rhs, const Phase &ph, int wg, int mode, const SbTile *tiles,
+                 const SbPiece *pieces, const SbXRange *xr, const float *dval, const float *sval,
+                 const uint16_t *sidx, const int32_t *srowptr, const int32_t *xmap, int32_t *queue,
+                 const float *const *xv, float *const *yv, const float *x, float *y, hipStream_t st,
+                 hipEvent_t e0, hipEvent_t e1)
 {
-    const sable::HostPlan &h = p->host;
-    const int ntiles = p->stats.ntiles;
-    if (ntiles == 0) return SABLE_OK;
-    const int mode = p->opts.mode;
-    const int grid = mode == SABLE_MODE_GRID ? ntiles : p->grid;
-    if (h.op == SABLE_OP_SPMV) {
+    const int n = ph.t1 - ph.t0;
+    if (n <= 0) return SABLE_OK;
+    const int grid = mode == SABLE_MODE_GRID ? n : std::min(ph.grid, n);
+    if (op == SABLE_OP_SPMV) {
         SbSpmvArgs a{};
-        a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
-        a.dval = p->d_dval; a.sval = p->d_sval; a.sidx = p->d_sidx; a.srowptr = p->d_srowptr;
-        a.xmap = p->d_xmap;
-        a.xvec = nullptr; a.yvec = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
-        a.tile_clock = p->clock_on ? p->d_clock : nullptr;
-        a.ntiles = ntiles; a.mode = mode;
-        return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream, e0, e1);
+        a.tiles = tiles + ph.t0; a.pieces = pieces; a.xr = xr;
+        a.dval = dval; a.sval = sval; a.sidx = sidx; a.srowptr = srowptr; a.xmap = xmap;
+        a.xvec = xv; a.yvec = yv; a.x0 = x; a.y0 = y; a.queue = queue;
+        a.ntiles = n; a.mode = mode;
+        return launch_raw(ph.rtc, ph.fn, ph.aot, grid, wg, &a, st, e0, e1);
     }
     SbSpmmArgs a{};
-    a.tiles = p->d_tiles; a.pieces = p->d_pieces; a.xr = p->d_xr;
-    a.dval = p->d_dval;
-    a.sval = p->d_sval; a.sidx = p->d_sidx; a.srowptr = p->d_srowptr;
-    a.xmat = nullptr; a.ymat = nullptr; a.x0 = x; a.y0 = y; a.queue = p->d_queue;
-    a.tile_clock = p->clock_on ? p->d_clock : nullptr;
-    a.ntiles = ntiles; a.mode = mode; a.n_rhs = h.n_rhs;
-    return launch_raw(p->rtc, p->fn, p->aot, grid, p->opts.wg, &a, p->stream, e0, e1);
+    a.tiles = tiles + ph.t0; a.pieces = pieces; a.xr = xr;
+    a.dval = dval; a.sval = sval; a.sidx = sidx; a.srowptr = srowptr;
+    a.xmat = xv; a.ymat = yv; a.x0 = x; a.y0 = y; a.queue = queue;
+    a.ntiles = n; a.mode = mode; a.n_rhs = n_rhs;
+    return launch_raw(ph.rtc, ph.fn, ph.aot, grid, wg, &a, st, e0, e1);
+}
+
+// Launches every phase of a plan in order on its stream.  With events: a
+// single launch carries them itself (kernel start and end); several
+// launches are bracketed by stream events (their gaps included).
+int plan_launch(SablePlan *p, const float *x, float *y, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+{
+    const size_t np = p->stats.ntiles > 0 ? p->phases.size() : 0;
+    const bool attached = np == 1;
+    if (e0 && !attached) HIPOK(hipEventRecord(e0, p->stream));
+    for (size_t i = 0; i < np; ++i) {
+        const Phase &ph = p->phases[i];
+        CHECKRC(launch_phase(p->host.op, p->host.n_rhs, ph, p->opts.wg, p->opts.mode, p->d_tiles, p->d_pieces,
+                             p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap, p->d_queue, nullptr,
+                             nullptr, x, y, p->stream, attached ? e0 : nullptr, attached ? e1 : nullptr));
+    }
+    if (e1 && !attached) HIPOK(hipEventRecord(e1, p->stream));
+    return SABLE_OK;
 }
 
 int ensure_flush(uint32_t **buf, int64_t *n, int64_t bytes)
@@ -609,15 +650,17 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     if (p->host_only) {
         SablePlanOpts no_rtc = o;
         no_rtc.use_rtc = 0;
-        bool r;
-        hipFunction_t f;
-        const void *ao;
-        select_kernel(p->host, no_rtc, r, f, ao, p->spec);   // spec only, nothing compiled
+        no_rtc.wg = 256;       // the spec is recorded, nothing is compiled
+        select_phases(p->host, no_rtc, p->stats.ntiles_mfma, p->phases);
+        for (auto &ph : p->phases) {
+            ph.spec[1] = o.wg;
+            ph.spec[2] = o.unroll;
+        }
         *out = p.release();
         return SABLE_OK;
     }
     if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
-    CHECKRC(select_kernel(p->host, o, p->rtc, p->fn, p->aot, p->spec));
+    CHECKRC(select_phases(p->host, o, p->stats.ntiles_mfma, p->phases));
     const sable::HostPlan &h = p->host;
     int64_t bytes = 0;
     CHECKRC(upload(&p->d_tiles, h.tiles, bytes));
@@ -632,9 +675,12 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     HIPOK(hipMemset(p->d_queue, 0, 2 * sizeof(int32_t)));
     HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     p->own_stream = true;
-    p->grid = occupancy_grid(p->rtc, p->fn, p->aot, o.wg, std::max(1, p->stats.ntiles), o.grid);
-    p->stats.rtc_used = p->rtc ? 1 : 0;
-    p->stats.grid = p->grid;
+    p->stats.rtc_used = 1;
+    for (auto &ph : p->phases) {
+        ph.grid = occupancy_grid(ph.rtc, ph.fn, ph.aot, o.wg, std::max(1, ph.t1 - ph.t0), o.grid);
+        p->stats.rtc_used = p->stats.rtc_used && ph.rtc;
+    }
+    p->stats.grid = p->phases.empty() ? 0 : p->phases[0].grid;
     p->stats.device_bytes = bytes;
     p->device_bytes = bytes;
     // the big arrays live on the device now; keep only descriptors on the host
@@ -664,7 +710,8 @@ int sable_plan_destroy(SablePlan *p)
 int sable_plan_kernel_spec(const SablePlan *p, int32_t *spec6)
 {
     if (!p || !spec6) return fail(SABLE_ERR_INVALID, "null argument");
-    for (int i = 0; i < 6; ++i) spec6[i] = p->spec[i];
+    if (p->phases.empty()) return fail(SABLE_ERR_INVALID, "plan has no kernel");
+    for (int i = 0; i < 6; ++i) spec6[i] = p->phases[0].spec[i];
     return SABLE_OK;
 }
 
@@ -838,6 +885,7 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     std::vector<SbPiece> pieces;
     std::vector<SbXRange> xr;
     std::vector<double> cost;
+    std::vector<int32_t> phase;
     int64_t dval_n = 0, sval_n = 0, srow_n = 0, xmap_n = 0;
     for (size_t m = 0; m < idx.size(); ++m) {
         SablePlan *p = plans[idx[m]];
@@ -864,6 +912,7 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
             if (tt.sp_map >= 0) tt.sp_map += (int32_t)srow_n;
             tiles.push_back(tt);
             cost.push_back(p->host.tile_cost[t]);
+            phase.push_back((int32_t)t < p->stats.ntiles_mfma ? 0 : 1);
         }
         dval_n += p->stats.dense_padded + p->stats.mfma_padded;
         sval_n += p->stats.sparse_padded;
@@ -875,7 +924,13 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     // global LPT order
     std::vector<int32_t> order(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    // MFMA tiles first (first launch), largest first in each phase
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        if (phase[a] != phase[b]) return phase[a] < phase[b];
+        return cost[a] > cost[b];
+    });
+    int32_t nm = 0;
+    for (int32_t ph : phase) nm += (ph == 0);
     std::vector<SbTile> sorted(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
     q.ntiles = (int32_t)sorted.size();
@@ -922,30 +977,19 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     view.n_rhs = q.n_rhs;
     view.tiles = sorted;
     view.pieces = pieces;
-    int32_t spec[6];
-    CHECKRC(select_kernel(view, p0->opts, q.rtc, q.fn, q.aot, spec));
-    q.grid = q.mode == SABLE_MODE_GRID ? q.ntiles
-                                       : occupancy_grid(q.rtc, q.fn, q.aot, q.wg, std::max(1, q.ntiles), grid);
+    CHECKRC(select_phases(view, p0->opts, nm, q.phases));
+    for (auto &ph : q.phases)
+        ph.grid = occupancy_grid(ph.rtc, ph.fn, ph.aot, q.wg, std::max(1, ph.t1 - ph.t0), grid);
     return SABLE_OK;
 }
 
 int launch_part(const GroupPart &q, hipStream_t st)
 {
-    if (q.ntiles == 0) return SABLE_OK;
-    if (q.op == SABLE_OP_SPMV) {
-        SbSpmvArgs a{};
-        a.tiles = q.d_tiles; a.pieces = q.d_pieces; a.xr = q.d_xr; a.dval = q.d_dval;
-        a.sval = q.d_sval; a.sidx = q.d_sidx; a.srowptr = q.d_srowptr; a.xmap = q.d_xmap;
-        a.xvec = q.d_xptr; a.yvec = q.d_yptr; a.queue = q.d_queue;
-        a.ntiles = q.ntiles; a.mode = q.mode;
-        return launch_raw(q.rtc, q.fn, q.aot, q.grid, q.wg, &a, st);
-    }
-    SbSpmmArgs a{};
-    a.tiles = q.d_tiles; a.pieces = q.d_pieces; a.xr = q.d_xr; a.dval = q.d_dval;
-    a.sval = q.d_sval; a.sidx = q.d_sidx; a.srowptr = q.d_srowptr;
-    a.xmat = q.d_xptr; a.ymat = q.d_yptr; a.queue = q.d_queue;
-    a.ntiles = q.ntiles; a.mode = q.mode; a.n_rhs = q.n_rhs;
-    return launch_raw(q.rtc, q.fn, q.aot, q.grid, q.wg, &a, st);
+    for (const Phase &ph : q.phases)
+        CHECKRC(launch_phase(q.op, q.n_rhs, ph, q.wg, q.mode, q.d_tiles, q.d_pieces, q.d_xr, q.d_dval, q.d_sval, q.d_sidx,
+                             q.d_srowptr, q.d_xmap, q.d_queue, q.d_xptr, q.d_yptr, nullptr, nullptr, st, nullptr,
+                             nullptr));
+    return SABLE_OK;
 }
 
 }  // namespace

@@ -34,7 +34,15 @@ struct SbTile {
     int32_t sp_waves;        // SpMV: wavefronts reserved for the sparse pieces (the
                              // last ones; the rest stream the dense pieces), or 0
                              // when every wavefront does both in turn
+    int32_t y_accum;         // SpMM: 1 = add to Y rows an MFMA tile wrote before
 };
+
+// SpMM MFMA tiles: a wavefront holds SB_MFMA_UNITS(N) 16-row tiles across
+// all N/16 column tiles (their FP32 accumulators plus the double-buffered A
+// fragments), which bounds a tile at SB_MFMA_MAX_ROWS(N, WG) rows (and 256).
+#define SB_MFMA_UNITS(N) ((N) >= 256 ? 1 : 2)
+#define SB_MFMA_MAX_ROWS(N, WG) \
+    (16 * ((WG) / 64) * SB_MFMA_UNITS(N) < 256 ? 16 * ((WG) / 64) * SB_MFMA_UNITS(N) : 256)
 
 // A panel piece of a tile: the tile's rows restricted to a group of blocks
 // whose columns (ncols <= the x staging capacity) are staged in LDS.

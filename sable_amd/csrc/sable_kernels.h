@@ -557,8 +557,15 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
 #pragma unroll
         for (int i = 0; i < RTMAX; ++i) {
             const int r = m.rbeg + i;
-            if (i < m.rt && r < hs)
-                *reinterpret_cast<float4 *>(Y + (size_t)(tile.row0 + r) * N + 4 * m.v) = acc[i];
+            if (i < m.rt && r < hs) {
+                float4 *dst = reinterpret_cast<float4 *>(Y + (size_t)(tile.row0 + r) * N + 4 * m.v);
+                float4 o = acc[i];
+                if (tile.y_accum) {      // the rows' MFMA tile wrote Y in the previous launch
+                    const float4 w = *dst;
+                    o = make_float4(w.x + o.x, w.y + o.y, w.z + o.z, w.w + o.w);
+                }
+                *dst = o;
+            }
         }
     }
     if (a.tile_clock) {

@@ -21,9 +21,9 @@
 // 40-element pitch so the B-operand reads (lane l: n = l%16, k = 8(l/16) ..
 // +7) are conflict-free 16-byte LDS reads.
 //
-// A tile may also hold FP32-dense and sparse pieces (other blocks of the
-// same block row); those run on the vector ALUs with the FP32 kernel's
-// thread mapping, and both partial results meet in an LDS Y tile.
+// MFMA tiles hold only MFMA pieces and write their Y rows; the block row's
+// FP32-dense and sparse pieces run afterwards, in a second launch of the
+// FP32 kernel whose tiles add to those rows.
 #pragma once
 
 #include "sable_kernels.h"
@@ -32,42 +32,33 @@ typedef __attribute__((ext_vector_type(8))) __bf16 sb_bf16x8;
 typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 
 #define SB_KP 40            // LDS pitch of a staged X column (bf16 elements)
-#define SB_MFMA_MAXROWS 64  // tile height limit of the MFMA kernel
 
 #ifndef SB_SPLIT
 #define SB_SPLIT 3
 #endif
 
-__device__ __forceinline__ uint32_t sb_bf16_bits(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (u >> 16) | 0x40u;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
-}
-
-__device__ __forceinline__ float sb_bf16_float(uint32_t h) { return __uint_as_float(h << 16); }
-
 __device__ __forceinline__ sb_bf16x8 sb_as_bf16x8(sb_u4 v) { return __builtin_bit_cast(sb_bf16x8, v); }
 
-// Splits 8 floats into SPLIT bf16x8 terms.
+// Splits 8 floats into SPLIT bf16x8 terms: t[0] = bf16(f) by the hardware
+// conversion (round to nearest even, v_cvt_pk_bf16_f32), then each term of
+// the exact FP32 residual.
 template <int SPLIT>
 __device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
 {
-    uint32_t w[SPLIT][4];
+    float r[8];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        float ra = f[2 * j], rb = f[2 * j + 1];
+    for (int j = 0; j < 8; ++j) r[j] = f[j];
 #pragma unroll
-        for (int s = 0; s < SPLIT; ++s) {
-            const uint32_t ha = sb_bf16_bits(ra), hb = sb_bf16_bits(rb);
-            w[s][j] = ha | (hb << 16);
-            ra -= sb_bf16_float(ha);
-            rb -= sb_bf16_float(hb);
+    for (int s = 0; s < SPLIT; ++s) {
+        sb_bf16x8 h;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) h[j] = (__bf16)r[j];
+        t[s] = h;
+        if (s + 1 < SPLIT) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] -= (float)h[j];
         }
     }
-#pragma unroll
-    for (int s = 0; s < SPLIT; ++s) t[s] = sb_as_bf16x8(sb_u4{w[s][0], w[s][1], w[s][2], w[s][3]});
 }
 
 // X staging of an MFMA piece, split in two phases so the global loads of
@@ -128,156 +119,145 @@ template <int WG, int NRHS, int SPLIT>
 struct SbMfmaCfg {
     static constexpr int NMAX = NRHS ? NRHS : 256;
     static constexpr int NTMAX = NMAX / 16;
-    // vector-path rows per thread for a 64-row tile: ceil(64 * (N/4) / WG)
-    static constexpr int RT0 = (SB_MFMA_MAXROWS * (NMAX / 4) + WG - 1) / WG;
-    static constexpr int RTMAX = RT0 < 1 ? 1 : RT0;
-    static constexpr int TERM_STRIDE = NMAX * SB_KP;            // bf16 elements per staged term
+    static constexpr int NW = WG / 64;
+    static constexpr int UNITS = SB_MFMA_UNITS(NMAX);            // row tiles per wavefront and pass
+    static constexpr int TERM_STRIDE = NMAX * SB_KP;             // bf16 elements per staged term
     static constexpr int UPT = (NMAX * 4 + WG - 1) / WG;         // X staging units per thread
-    static constexpr int BSTAGE_BYTES = SPLIT * TERM_STRIDE * 2;
-    static constexpr int YTILE_BYTES = SB_MFMA_MAXROWS * NMAX * 4;
-    static constexpr int SMEM0 = BSTAGE_BYTES > YTILE_BYTES ? BSTAGE_BYTES : YTILE_BYTES;
-    static constexpr int SMEM_BYTES = SMEM0 > 8 * SB_SPMM_SPCHUNK ? SMEM0 : 8 * SB_SPMM_SPCHUNK;
+    static constexpr int SMEM_BYTES = SPLIT * TERM_STRIDE * 2;
 };
 
+// One MFMA tile (only MFMA pieces; it writes its Y rows).  Work units are
+// (16-row tile mi, column group gi) pairs: ng column groups of nt/ng
+// 16-column tiles, ng the smallest power of two that gives every wavefront
+// a unit when the tile is short.  Wavefront w takes units w, w + NW, ...,
+// UNITS of them per pass (one pass for tiles the host cut to
+// SB_MFMA_MAX_ROWS; more only if a generic kernel meets a taller tile).  Per
+// 32-column k-step the workgroup stages the X rows once (split into bf16
+// terms in LDS) and every unit multiplies its A fragment against them; the A
+// fragments and X values of k-step s+1 are loaded while k-step s computes.
 template <int WG, int NRHS, int SPLIT>
 __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem)
 {
     using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
+    constexpr int NW = Cfg::NW;
+    constexpr int UN = Cfg::UNITS;
     sb_clock(a.tile_clock, t, 0);
     const int N = NRHS ? NRHS : a.n_rhs;
     const int nt = N >> 4;
     const SbTile tile = a.tiles[t];
     const int hs = tile.nrows;
     const int mt = (hs + 15) >> 4;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    constexpr int NWAVES = WG / 64;
+    // ng divides NW (both powers of two, ng <= NW), so the units of one
+    // wavefront (w, w + NW, ...) share the column group w % ng
     int ng = 1;
-    while (ng * 2 <= nt && mt * ng * 2 <= NWAVES) ng <<= 1;
-    const int mi = wave / ng;
-    const int gi = wave - mi * ng;
-    const bool mactive = mi < mt;
+    while (ng * 2 <= nt && mt * ng < NW) ng <<= 1;
+    const int gj0 = (wave % ng) * (nt / ng);
     const int ntw = nt / ng;
-    const int j0 = gi * ntw;
+    const int units = mt * ng;
+    const int mine = units > wave ? (units - wave + NW - 1) / NW : 0;   // units of this wavefront
+    const int passes = (mine + UN - 1) / UN;
     const float *__restrict__ X = a.xmat ? a.xmat[tile.mat] : a.x0;
     float *__restrict__ Y = a.ymat ? a.ymat[tile.mat] : a.y0;
     uint16_t *B = reinterpret_cast<uint16_t *>(smem);
-    float *ytile = reinterpret_cast<float *>(smem);
+    const int nb = lane & 15, kb = (lane >> 4) * 8;
 
-    sb_f32x4 acc[Cfg::NTMAX];
+    // every wavefront runs the same number of passes (the staging barriers)
+    int all_passes = (((units + NW - 1) / NW) + UN - 1) / UN;
+    if (all_passes < 1) all_passes = 1;
+    for (int pass = 0; pass < all_passes; ++pass) {
+        sb_f32x4 acc[UN][Cfg::NTMAX];
 #pragma unroll
-    for (int j = 0; j < Cfg::NTMAX; ++j) acc[j] = sb_f32x4{0.f, 0.f, 0.f, 0.f};
-    const SbVecMap vm = sb_vec_map(hs, N, WG);
-    float4 vacc[Cfg::RTMAX];
+        for (int u = 0; u < UN; ++u)
 #pragma unroll
-    for (int i = 0; i < Cfg::RTMAX; ++i) vacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    bool has_mfma = false, has_vec = false;
-
-    for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
-        const SbPiece pc = a.pieces[pi];
-        __syncthreads();
-        sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
-        __syncthreads();
-        if (pc.kind == SB_PIECE_MFMA) {
-            has_mfma = true;
+            for (int j = 0; j < Cfg::NTMAX; ++j) acc[u][j] = sb_f32x4{0.f, 0.f, 0.f, 0.f};
+        // this pass's units: index i -> unit wave + NW * (pass * UN + i)
+        int umi[UN];
+        bool uact[UN];
+#pragma unroll
+        for (int i = 0; i < UN; ++i) {
+            const int k = pass * UN + i;
+            const int u = wave + NW * k;
+            uact[i] = k < mine && pass < passes;
+            umi[i] = uact[i] ? u / ng : 0;
+        }
+        for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
+            const SbPiece pc = a.pieces[pi];
+            if (pc.kind != SB_PIECE_MFMA) continue;          // never built into an MFMA tile
+            __syncthreads();
+            sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
+            __syncthreads();
             const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
             float xf[Cfg::UPT][8];
-            sb_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
-            sb_f4 n0 = a0, n1 = a1;
+            sb_f4 an[UN][2];
             sb_fetch_b<WG, Cfg::UPT>(X, N, xrow, pc.ncols, 0, xf);
-            if (mactive) {
-                const int off = (mi * pc.ksteps * 64 + lane) * 2;
-                n0 = __builtin_nontemporal_load(&A[off]);
-                n1 = __builtin_nontemporal_load(&A[off + 1]);
+#pragma unroll
+            for (int i = 0; i < UN; ++i) {
+                const int off = (umi[i] * pc.ksteps * 64 + lane) * 2;
+                an[i][0] = uact[i] ? __builtin_nontemporal_load(&A[off]) : sb_f4{0.f, 0.f, 0.f, 0.f};
+                an[i][1] = uact[i] ? __builtin_nontemporal_load(&A[off + 1]) : sb_f4{0.f, 0.f, 0.f, 0.f};
             }
             for (int ks = 0; ks < pc.ksteps; ++ks) {
                 __syncthreads();                       // B free: previous k-step consumed
                 sb_store_b<WG, Cfg::UPT, SPLIT>(N, xf, B, Cfg::TERM_STRIDE);
-                a0 = n0;
-                a1 = n1;
+                sb_f4 ac[UN][2];
+#pragma unroll
+                for (int i = 0; i < UN; ++i) {
+                    ac[i][0] = an[i][0];
+                    ac[i][1] = an[i][1];
+                }
                 __syncthreads();
                 if (ks + 1 < pc.ksteps) {              // next k-step's loads overlap this one's MFMAs
                     sb_fetch_b<WG, Cfg::UPT>(X, N, xrow, pc.ncols, ks + 1, xf);
-                    if (mactive) {
-                        const int off = ((mi * pc.ksteps + ks + 1) * 64 + lane) * 2;
-                        n0 = __builtin_nontemporal_load(&A[off]);
-                        n1 = __builtin_nontemporal_load(&A[off + 1]);
-                    }
-                }
-                if (mactive) {
-                    const float af[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-                    sb_bf16x8 at[SPLIT];
-                    sb_split8<SPLIT>(af, at);
-                    const int nb = lane & 15, kb = (lane >> 4) * 8;
 #pragma unroll
-                    for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
-                        if (jj < ntw) {
-                            const int n = (j0 + jj) * 16 + nb;
-                            sb_bf16x8 bt[SPLIT];
-#pragma unroll
-                            for (int s = 0; s < SPLIT; ++s)
-                                bt[s] = *reinterpret_cast<const sb_bf16x8 *>(B + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
-                            acc[jj] = sb_mfma_terms<SPLIT>(at, bt, acc[jj]);
+                    for (int i = 0; i < UN; ++i) {
+                        const int off = ((umi[i] * pc.ksteps + ks + 1) * 64 + lane) * 2;
+                        if (uact[i]) {
+                            an[i][0] = __builtin_nontemporal_load(&A[off]);
+                            an[i][1] = __builtin_nontemporal_load(&A[off + 1]);
                         }
                     }
                 }
-            }
-        } else {
-            has_vec = true;
-            // LDS scratch for the sparse entries: the B staging area is free here
-            sb_spmm_vec_piece<WG, Cfg::RTMAX>(a, pc, hs, N, vm, xrow, X, vacc, reinterpret_cast<float *>(smem),
-                                              reinterpret_cast<int *>(smem) + SB_SPMM_SPCHUNK);
-        }
-    }
-
-    // D fragment of n-tile j: lane holds rows 16*mi + 4*(lane/16) + i, column 16*j + lane%16
-    const int rbase = mi * 16 + 4 * (lane >> 4);
-    if (has_mfma && !has_vec) {
-        if (mactive) {
+                // A terms of every unit, then each B fragment read once from
+                // LDS and applied to all units (units of a wavefront share
+                // their column group)
+                sb_bf16x8 at[UN][SPLIT];
 #pragma unroll
-            for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
-                if (jj < ntw) {
-                    const int col = (j0 + jj) * 16 + (lane & 15);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int r = rbase + i;
-                        if (r < hs) Y[(size_t)(tile.row0 + r) * N + col] = acc[jj][i];
-                    }
+                for (int i = 0; i < UN; ++i) {
+                    const float af[8] = {ac[i][0].x, ac[i][0].y, ac[i][0].z, ac[i][0].w,
+                                         ac[i][1].x, ac[i][1].y, ac[i][1].z, ac[i][1].w};
+                    sb_split8<SPLIT>(af, at[i]);
                 }
-            }
-        }
-    } else {
-        if (has_mfma) {
-            __syncthreads();
-            if (mactive) {
 #pragma unroll
                 for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
                     if (jj < ntw) {
-                        const int col = (j0 + jj) * 16 + (lane & 15);
+                        const int n = (gj0 + jj) * 16 + nb;
+                        sb_bf16x8 bt[SPLIT];
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const int r = rbase + i;
-                            if (r < hs) ytile[r * N + col] = acc[jj][i];
-                        }
+                        for (int s = 0; s < SPLIT; ++s)
+                            bt[s] = *reinterpret_cast<const sb_bf16x8 *>(B + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
+#pragma unroll
+                        for (int i = 0; i < UN; ++i)
+                            if (uact[i]) acc[i][jj] = sb_mfma_terms<SPLIT>(at[i], bt, acc[i][jj]);
                     }
                 }
             }
-            __syncthreads();
         }
-        if (vm.active) {
+        // D fragment of column tile j: lane holds rows 16*mi + 4*(lane/16) + r,
+        // column 16*j + lane%16
 #pragma unroll
-            for (int i = 0; i < Cfg::RTMAX; ++i) {
-                const int r = vm.rbeg + i;
-                if (i < vm.rt && r < hs) {
-                    float4 o = vacc[i];
-                    if (has_mfma) {
-                        const float4 m4 = *reinterpret_cast<const float4 *>(ytile + r * N + 4 * vm.v);
-                        o.x += m4.x;
-                        o.y += m4.y;
-                        o.z += m4.z;
-                        o.w += m4.w;
-                    }
-                    *reinterpret_cast<float4 *>(Y + (size_t)(tile.row0 + r) * N + 4 * vm.v) = o;
+        for (int i = 0; i < UN; ++i) {
+            if (!uact[i]) continue;
+            const int j0 = gj0;
+            const int rbase = umi[i] * 16 + 4 * (lane >> 4);
+#pragma unroll
+            for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
+                if (jj < ntw) {
+                    const int col = (j0 + jj) * 16 + nb;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (rbase + r < hs) Y[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
                 }
             }
         }

@@ -74,10 +74,22 @@ def reconstruct(plan):
     srow = plan.export("srowptr")
     M = np.zeros((plan.nrows, plan.ncols), dtype=np.float64)
     Mb = np.zeros_like(M)   # bf16 hi+lo parts
-    cover = np.zeros(plan.nrows, dtype=np.int64)
-    for t in tiles:
+    cover = np.zeros(plan.nrows, dtype=np.int64)      # tiles that write each Y row
+    mfma_rows = np.zeros(plan.nrows, dtype=bool)
+    nm = plan.stats["ntiles_mfma"]
+    for ti, t in enumerate(tiles):
         r0, hs = int(t["row0"]), int(t["nrows"])
-        cover[r0:r0 + hs] += 1
+        kinds = {int(pieces[k]["kind"]) for k in range(t["piece_begin"], t["piece_end"])}
+        if ti < nm:
+            # SpMM MFMA tiles come first, hold only MFMA pieces, 16-row multiples
+            assert kinds <= {2} and t["y_accum"] == 0 and (r0 - 0) >= 0
+            mfma_rows[r0:r0 + hs] = True
+        else:
+            assert 2 not in kinds
+        if t["y_accum"]:
+            assert ti >= nm and np.all(mfma_rows[r0:r0 + hs])   # adds to rows an MFMA tile wrote
+        else:
+            cover[r0:r0 + hs] += 1
         for p in pieces[t["piece_begin"]:t["piece_end"]]:
             cols = np.concatenate([np.arange(c0, c0 + n) for c0, n in xr[p["xr_begin"]:p["xr_end"]]])
             assert len(cols) == p["ncols"]
@@ -217,7 +229,14 @@ def test_host_plan_mfma_layout():
     assert np.all(cover == 1)
     assert np.array_equal(M + Mb, D.astype(np.float32).astype(np.float64))
     assert np.count_nonzero(Mb) > 0
-    assert all(t["nrows"] <= 64 for t in plan.export("tiles"))
+    tiles = plan.export("tiles")
+    nm = plan.stats["ntiles_mfma"]
+    assert nm > 0
+    # MFMA tiles: at most 16 * (wg/64) * 2 rows (two 16-row units per wavefront), 16-row multiples
+    # except at a block row's end; the vector-route tiles keep the FP32 kernel's 64-row bound
+    wg = plan.stats["wg"]
+    assert all(t["nrows"] <= min(256, 16 * (wg // 64) * 2) for t in tiles[:nm])
+    assert all(t["nrows"] <= 64 for t in tiles[nm:])
 
 
 def test_bad_vbr_rejected():
