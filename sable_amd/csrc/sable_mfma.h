@@ -64,32 +64,51 @@ __device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
 // X staging of an MFMA piece, split in two phases so the global loads of
 // k-step s+1 are in flight while k-step s computes: sb_fetch_b loads the
 // thread's X values of a k-step into registers, sb_store_b splits them and
-// writes B[s][n * SB_KP + k] (k < 32, s < SPLIT) to LDS.
-template <int WG, int UPT>
-__device__ __forceinline__ void sb_fetch_b(const float *__restrict__ X, int N, const int *xrow, int ncols, int ks,
+// writes B[s][n * SB_KP + k] (k < 32, s < SPLIT) to LDS.  Unit u of the
+// k-step is (k group kg = u / N of 8 rows, column n = u % N); its 8 X rows
+// come from xrow in two 16-byte LDS reads, and every load is unconditional
+// (rows past the piece read column n of X row 0 and are replaced by 0), so
+// there is no divergent control flow.  X is read through a global-address
+// pointer with 32-bit offsets.
+typedef const __attribute__((address_space(1))) float *sb_gfloat;
+
+template <int WG, int UPT, int NRHS>
+__device__ __forceinline__ void sb_fetch_b(const float *X, int N, const int *xrow, int ncols, int ks,
                                            float (*f)[8])
 {
+    const sb_gfloat gX = (sb_gfloat)X;
+    constexpr bool EXACT = NRHS != 0 && UPT * WG == NRHS * 4;   // every unit in range
     const int units = N * 4;
 #pragma unroll
     for (int uu = 0; uu < UPT; ++uu) {
         const int u = threadIdx.x + uu * WG;
-        const int kg = u / N;
-        const int n = u - kg * N;
+        const bool uok = EXACT || u < units;
+        const int uc = uok ? u : 0;
+        const int kg = uc / N;
+        const int n = uc - kg * N;
         const int k0 = ks * 32 + kg * 8;
+        const int4 ra = *reinterpret_cast<const int4 *>(xrow + k0);
+        const int4 rb = *reinterpret_cast<const int4 *>(xrow + k0 + 4);
+        const int rows[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            f[uu][j] = (u < units && k0 + j < ncols) ? X[(size_t)xrow[k0 + j] * N + n] : 0.f;
+        for (int j = 0; j < 8; ++j) {
+            const bool ok = uok && k0 + j < ncols;
+            const uint32_t off = (uint32_t)(ok ? rows[j] : 0) * (uint32_t)N + (uint32_t)n;
+            const float v = gX[off];
+            f[uu][j] = ok ? v : 0.f;
+        }
     }
 }
 
-template <int WG, int UPT, int SPLIT>
+template <int WG, int UPT, int SPLIT, int NRHS>
 __device__ __forceinline__ void sb_store_b(int N, float (*f)[8], uint16_t *B, int term_stride)
 {
+    constexpr bool EXACT = NRHS != 0 && UPT * WG == NRHS * 4;
     const int units = N * 4;
 #pragma unroll
     for (int uu = 0; uu < UPT; ++uu) {
         const int u = threadIdx.x + uu * WG;
-        if (u >= units) continue;
+        if (!EXACT && u >= units) continue;
         const int kg = u / N;
         const int n = u - kg * N;
         sb_bf16x8 t[SPLIT];
@@ -191,7 +210,7 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
             const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
             float xf[Cfg::UPT][8];
             sb_f4 an[UN][2];
-            sb_fetch_b<WG, Cfg::UPT>(X, N, xrow, pc.ncols, 0, xf);
+            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf);
 #pragma unroll
             for (int i = 0; i < UN; ++i) {
                 const int off = (umi[i] * pc.ksteps * 64 + lane) * 2;
@@ -200,7 +219,7 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
             }
             for (int ks = 0; ks < pc.ksteps; ++ks) {
                 __syncthreads();                       // B free: previous k-step consumed
-                sb_store_b<WG, Cfg::UPT, SPLIT>(N, xf, B, Cfg::TERM_STRIDE);
+                sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, xf, B, Cfg::TERM_STRIDE);
                 sb_f4 ac[UN][2];
 #pragma unroll
                 for (int i = 0; i < UN; ++i) {
@@ -209,7 +228,7 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
                 }
                 __syncthreads();
                 if (ks + 1 < pc.ksteps) {              // next k-step's loads overlap this one's MFMAs
-                    sb_fetch_b<WG, Cfg::UPT>(X, N, xrow, pc.ncols, ks + 1, xf);
+                    sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1, xf);
 #pragma unroll
                     for (int i = 0; i < UN; ++i) {
                         const int off = ((umi[i] * pc.ksteps + ks + 1) * 64 + lane) * 2;
@@ -272,7 +291,7 @@ template <int WG, int NRHS, int XCAP, int SPLIT>
 __device__ __forceinline__ void sb_spmm_mfma_body(const SbSpmmArgs &a)
 {
     using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
-    __shared__ int xrow[XCAP + 4];
+    __shared__ __attribute__((aligned(16))) int xrow[XCAP + 32];   // k-steps read up to 31 past ncols
     __shared__ __attribute__((aligned(16))) unsigned char smem[Cfg::SMEM_BYTES];
     __shared__ int next_tile;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
