@@ -30,9 +30,9 @@ SPMV_SPACE = {
     "wg": (256, 512),
 }
 SPMM_SPACE = {
-    "tile_bytes": (32 << 10, 64 << 10, 128 << 10),
-    "mode": (0, 2),
-    "wg": (256,),
+    "tile_bytes": (256 << 10, 512 << 10, 1 << 20, 2 << 20),
+    "mode": (0,),
+    "wg": (256, 512),
 }
 
 

@@ -34,7 +34,7 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     if (o.op == SABLE_OP_SPMV) o.n_rhs = 1;
     if (o.wg <= 0) o.wg = 256;
     if (o.xcap <= 0) o.xcap = 4095;
-    if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 128 * 1024 : 64 * 1024;
+    if (o.tile_bytes <= 0) o.tile_bytes = (o.op == SABLE_OP_SPMV) ? 128 * 1024 : 512 * 1024;
     if (o.max_seg_rows <= 0) o.max_seg_rows = o.wg;
     if (o.min_seg_rows <= 0) o.min_seg_rows = (o.op == SABLE_OP_SPMV) ? 8 : 16;
     if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_GRID;
