@@ -85,7 +85,8 @@ typedef struct {
                                  (FP32-level, default); 2 = hi+lo, 3 passes */
     int32_t min_tiles;        /* tiles wanted at least: while a plan has fewer, its
                                  tile_bytes is halved (down to 16 KiB) so small
-                                 matrices still fill the device; 0 = 1024 */
+                                 matrices still fill the device; 0 = 1024 for
+                                 SpMV, 256 for SpMM */
 } SablePlanOpts;
 
 typedef struct {

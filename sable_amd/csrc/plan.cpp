@@ -40,7 +40,9 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     if (o.mode < 0 || o.mode > 2) o.mode = SABLE_MODE_GRID;
     if (o.unroll <= 0) o.unroll = 4;
     if (o.mfma_split != 2) o.mfma_split = 3;
-    if (o.min_tiles <= 0) o.min_tiles = 1024;
+    // SpMV tiles are light (stream-bound); SpMM tiles carry MFMA work and
+    // gain from tall tiles (each staged X k-step serves every row)
+    if (o.min_tiles <= 0) o.min_tiles = (o.op == SABLE_OP_SPMV) ? 1024 : 256;
     return o;
 }
 

@@ -67,3 +67,55 @@ def test_training_lowers_loss(mods):
     print("case: mlp training", flush=True)
     hist = net.train(X, T, M, steps=300, batch=512, lr=1e-2, log_every=100)
     assert hist[-1][1] < 0.5 * hist[0][1]
+
+
+def _c3_like(n, seed):
+    from sable_amd import synth
+    vbr, _, _ = synth.mixed_vbr(n, n // 100, 32, 192, 0.02, 0.7, (0.9, 1.0), (0.02, 0.1), seed)
+    X = np.random.default_rng(seed + 1).uniform(-1, 1, (n, 128)).astype(np.float32)
+    return vbr, X
+
+
+def test_trained_router_gains_over_fixed_threshold(mods):
+    """What the trained network gains: on a 30k x 30k SpMM (N = 128) shaped
+    like the 50k workload, the committed router's plan is within 1e-5 of the
+    FP64 oracle like the fixed-threshold plan (T = 15, dense blocks on
+    MFMA) and at least 10% faster cold (median of 15 launches)."""
+    rt, _ = mods
+    from oracle import c_oracle
+    from sable_amd.convert import threshold_routes, ROUTE_DENSE, ROUTE_DENSE_MFMA
+    from sable_amd.router import Router
+    assert Router.available()
+    vbr, X = _c3_like(30000, 21)
+    fixed = threshold_routes(vbr.val, vbr.indx, 15)
+    fixed = np.where(fixed == ROUTE_DENSE, ROUTE_DENSE_MFMA, fixed).astype(np.uint8)
+    learned = Router.load().route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                         vbr.bpntre, 128)
+    Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                             vbr.bpntrb, vbr.bpntre, X)
+    Xd = rt.DeviceArray.from_numpy(X)
+    Yd = rt.DeviceArray((X.shape[0], 128))
+    out = {}
+    for name, routes in (("fixed", fixed), ("learned", learned)):
+        print(f"case: router gain, {name} plan", flush=True)
+        p = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=128)
+        t = float(np.median(p.time(Xd, Yd, warmup=3, reps=15, flush_bytes=1 << 30)))
+        err = c_oracle.scaled_error(Yd.numpy().ravel(), Y64.ravel(), AY.ravel())["componentwise"]
+        out[name] = (t, err)
+        p.close()
+    print(f"fixed {out['fixed'][0]:.4f} ms, learned {out['learned'][0]:.4f} ms", flush=True)
+    assert out["fixed"][1] <= 1e-5 and out["learned"][1] <= 1e-5
+    assert out["learned"][0] < 0.9 * out["fixed"][0]
+
+
+def test_choose_plan_keeps_the_faster_plan(mods):
+    rt, _ = mods
+    from sable_amd import synth
+    from sable_amd.router import Router, choose_plan
+    vbr, x, _ = synth.workload_like_c2(20000, seed=31)
+    print("case: choose_plan on a 20k SpMV", flush=True)
+    plan, rep = choose_plan(vbr, x, router=Router.load(), reps=10)
+    assert rep["learned_within_tol"]
+    assert rep["kept"] == ("learned" if rep["learned_ms"] < rep["fixed_ms"] else "fixed")
+    y = plan.spmv(x)
+    assert np.all(np.isfinite(y))
