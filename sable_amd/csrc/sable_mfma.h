@@ -40,8 +40,11 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 __device__ __forceinline__ sb_bf16x8 sb_as_bf16x8(sb_u4 v) { return __builtin_bit_cast(sb_bf16x8, v); }
 
 // Splits 8 floats into SPLIT bf16x8 terms: t[0] = bf16(f) by the hardware
-// conversion (round to nearest even, v_cvt_pk_bf16_f32), then each term of
-// the exact FP32 residual.
+// conversion (round to nearest even, one v_cvt_pk_bf16_f32 per pair), then
+// each term of the exact FP32 residual.  The bf16 pair p converts back to
+// FP32 without rounding: element 0 is p << 16, element 1 is p & 0xffff0000.
+typedef __attribute__((ext_vector_type(2))) __bf16 sb_bf16x2;
+
 template <int SPLIT>
 __device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
 {
@@ -50,25 +53,31 @@ __device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
     for (int j = 0; j < 8; ++j) r[j] = f[j];
 #pragma unroll
     for (int s = 0; s < SPLIT; ++s) {
-        sb_bf16x8 h;
+        sb_u4 w;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) h[j] = (__bf16)r[j];
-        t[s] = h;
-        if (s + 1 < SPLIT) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) r[j] -= (float)h[j];
+        for (int q = 0; q < 4; ++q) {
+            const sb_bf16x2 h = {(__bf16)r[2 * q], (__bf16)r[2 * q + 1]};
+            uint32_t p = __builtin_bit_cast(uint32_t, h);
+            asm("" : "+v"(p));   // empty: keeps the compiler from re-deriving p << 16 by a second conversion
+            w[q] = p;
+            if (s + 1 < SPLIT) {
+                r[2 * q] -= __builtin_bit_cast(float, p << 16);
+                r[2 * q + 1] -= __builtin_bit_cast(float, p & 0xffff0000u);
+            }
         }
+        t[s] = sb_as_bf16x8(w);
     }
 }
 
-// X staging of an MFMA piece, split in two phases so the global loads of
-// k-step s+1 are in flight while k-step s computes: sb_fetch_b loads the
-// thread's X values of a k-step into registers, sb_store_b splits them and
-// writes B[s][n * SB_KP + k] (k < 32, s < SPLIT) to LDS.  Unit u of the
-// k-step is (k group kg = u / N of 8 rows, column n = u % N); its 8 X rows
-// come from xrow in two 16-byte LDS reads, and every load is unconditional
-// (rows past the piece read column n of X row 0 and are replaced by 0), so
-// there is no divergent control flow.  X is read through a global-address
+// X staging of an MFMA piece, split in two phases so the global loads of a
+// later k-step are in flight while the current one computes: sb_fetch_b
+// loads the thread's X values of a k-step into registers, sb_store_b zeroes
+// the rows past the piece, splits the values and writes B[s][n * SB_KP + k]
+// (k < 32, s < SPLIT) to LDS.  Unit u of the k-step is (k group kg = u / N
+// of 8 rows, column n = u % N); its 8 X rows come from xrow in two 16-byte
+// LDS reads, and every load is unconditional (rows past the piece read
+// column n of X row 0), so there is no divergent control flow and no use of
+// a loaded value before sb_store_b.  X is read through a global-address
 // pointer with 32-bit offsets.
 typedef const __attribute__((address_space(1))) float *sb_gfloat;
 
@@ -94,14 +103,13 @@ __device__ __forceinline__ void sb_fetch_b(const float *X, int N, const int *xro
         for (int j = 0; j < 8; ++j) {
             const bool ok = uok && k0 + j < ncols;
             const uint32_t off = (uint32_t)(ok ? rows[j] : 0) * (uint32_t)N + (uint32_t)n;
-            const float v = gX[off];
-            f[uu][j] = ok ? v : 0.f;
+            f[uu][j] = gX[off];
         }
     }
 }
 
 template <int WG, int UPT, int SPLIT, int NRHS>
-__device__ __forceinline__ void sb_store_b(int N, float (*f)[8], uint16_t *B, int term_stride)
+__device__ __forceinline__ void sb_store_b(int N, int ncols, int ks, float (*f)[8], uint16_t *B, int term_stride)
 {
     constexpr bool EXACT = NRHS != 0 && UPT * WG == NRHS * 4;
     const int units = N * 4;
@@ -111,8 +119,12 @@ __device__ __forceinline__ void sb_store_b(int N, float (*f)[8], uint16_t *B, in
         if (!EXACT && u >= units) continue;
         const int kg = u / N;
         const int n = u - kg * N;
+        const int k0 = ks * 32 + kg * 8;
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = k0 + j < ncols ? f[uu][j] : 0.f;
         sb_bf16x8 t[SPLIT];
-        sb_split8<SPLIT>(f[uu], t);
+        sb_split8<SPLIT>(v, t);
 #pragma unroll
         for (int s = 0; s < SPLIT; ++s)
             *reinterpret_cast<sb_bf16x8 *>(B + s * term_stride + n * SB_KP + kg * 8) = t[s];
@@ -142,7 +154,8 @@ struct SbMfmaCfg {
     static constexpr int UNITS = SB_MFMA_UNITS(NMAX);            // row tiles per wavefront and pass
     static constexpr int TERM_STRIDE = NMAX * SB_KP;             // bf16 elements per staged term
     static constexpr int UPT = (NMAX * 4 + WG - 1) / WG;         // X staging units per thread
-    static constexpr int SMEM_BYTES = SPLIT * TERM_STRIDE * 2;
+    static constexpr int BUF_STRIDE = SPLIT * TERM_STRIDE;        // one staged k-step (bf16 elements)
+    static constexpr int SMEM_BYTES = 2 * BUF_STRIDE * 2;          // double-buffered
 };
 
 // One MFMA tile (only MFMA pieces; it writes its Y rows).  Work units are
@@ -152,8 +165,11 @@ struct SbMfmaCfg {
 // UNITS of them per pass (one pass for tiles the host cut to
 // SB_MFMA_MAX_ROWS; more only if a generic kernel meets a taller tile).  Per
 // 32-column k-step the workgroup stages the X rows once (split into bf16
-// terms in LDS) and every unit multiplies its A fragment against them; the A
-// fragments and X values of k-step s+1 are loaded while k-step s computes.
+// terms in one of two LDS buffers) and every unit multiplies its A fragment
+// against them.  The X values of k-step s+1 are split into the other buffer
+// after k-step s's MFMAs are issued (the two overlap), their global loads
+// were issued a k-step earlier, and one barrier per k-step orders the
+// buffers.
 template <int WG, int NRHS, int SPLIT>
 __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem)
 {
@@ -204,49 +220,52 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
         for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
             const SbPiece pc = a.pieces[pi];
             if (pc.kind != SB_PIECE_MFMA) continue;          // never built into an MFMA tile
-            __syncthreads();
+            __syncthreads();                               // previous piece done with xrow and B
             sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
             __syncthreads();
+            const int K = pc.ksteps;
             const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
+            // pipeline: B of k-step s in LDS buffer s & 1.  The A fragments
+            // of k-step s + 1 are loaded while k-step s computes; the X values
+            // of k-step s + 1 were loaded during k-step s - 1 and are split
+            // into the free buffer after k-step s's MFMAs are issued (the two
+            // overlap).  One barrier per k-step.
             float xf[Cfg::UPT][8];
             sb_f4 an[UN][2];
-            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf);
 #pragma unroll
             for (int i = 0; i < UN; ++i) {
-                const int off = (umi[i] * pc.ksteps * 64 + lane) * 2;
-                an[i][0] = uact[i] ? __builtin_nontemporal_load(&A[off]) : sb_f4{0.f, 0.f, 0.f, 0.f};
-                an[i][1] = uact[i] ? __builtin_nontemporal_load(&A[off + 1]) : sb_f4{0.f, 0.f, 0.f, 0.f};
+                const int off = (umi[i] * K * 64 + lane) * 2;
+                an[i][0] = __builtin_nontemporal_load(&A[off]);
+                an[i][1] = __builtin_nontemporal_load(&A[off + 1]);
             }
-            for (int ks = 0; ks < pc.ksteps; ++ks) {
-                __syncthreads();                       // B free: previous k-step consumed
-                sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, xf, B, Cfg::TERM_STRIDE);
-                sb_f4 ac[UN][2];
-#pragma unroll
-                for (int i = 0; i < UN; ++i) {
-                    ac[i][0] = an[i][0];
-                    ac[i][1] = an[i][1];
-                }
-                __syncthreads();
-                if (ks + 1 < pc.ksteps) {              // next k-step's loads overlap this one's MFMAs
-                    sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1, xf);
-#pragma unroll
-                    for (int i = 0; i < UN; ++i) {
-                        const int off = ((umi[i] * pc.ksteps + ks + 1) * 64 + lane) * 2;
-                        if (uact[i]) {
-                            an[i][0] = __builtin_nontemporal_load(&A[off]);
-                            an[i][1] = __builtin_nontemporal_load(&A[off + 1]);
-                        }
-                    }
-                }
-                // A terms of every unit, then each B fragment read once from
-                // LDS and applied to all units (units of a wavefront share
-                // their column group)
+            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf);
+            sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, 0, xf, B, Cfg::TERM_STRIDE);
+            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, K > 1 ? 1 : 0, xf);
+            __syncthreads();
+            for (int ks = 0; ks < K; ++ks) {
+                const uint16_t *Bc = B + (ks & 1) * Cfg::BUF_STRIDE;
+                // A terms of every unit (the registers are then reloaded with
+                // k-step ks + 1), then each B fragment read once from LDS and
+                // applied to all units (units of a wavefront share their
+                // column group)
                 sb_bf16x8 at[UN][SPLIT];
 #pragma unroll
                 for (int i = 0; i < UN; ++i) {
-                    const float af[8] = {ac[i][0].x, ac[i][0].y, ac[i][0].z, ac[i][0].w,
-                                         ac[i][1].x, ac[i][1].y, ac[i][1].z, ac[i][1].w};
+                    const float af[8] = {an[i][0].x, an[i][0].y, an[i][0].z, an[i][0].w,
+                                         an[i][1].x, an[i][1].y, an[i][1].z, an[i][1].w};
                     sb_split8<SPLIT>(af, at[i]);
+                }
+                {
+                    // unconditional loads (the last k-step reloads itself,
+                    // inactive units read unit 0) keep the control flow
+                    // uniform, so the waits on them stay precise
+                    const int kn = ks + 1 < K ? ks + 1 : ks;
+#pragma unroll
+                    for (int i = 0; i < UN; ++i) {
+                        const int off = ((umi[i] * K + kn) * 64 + lane) * 2;
+                        an[i][0] = __builtin_nontemporal_load(&A[off]);
+                        an[i][1] = __builtin_nontemporal_load(&A[off + 1]);
+                    }
                 }
 #pragma unroll
                 for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
@@ -255,12 +274,18 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
                         sb_bf16x8 bt[SPLIT];
 #pragma unroll
                         for (int s = 0; s < SPLIT; ++s)
-                            bt[s] = *reinterpret_cast<const sb_bf16x8 *>(B + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
+                            bt[s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
 #pragma unroll
                         for (int i = 0; i < UN; ++i)
                             if (uact[i]) acc[i][jj] = sb_mfma_terms<SPLIT>(at[i], bt, acc[i][jj]);
                     }
                 }
+                // stage k-step ks + 1 into the free buffer (after the last
+                // k-step this writes zeros nobody reads), then load ks + 2
+                sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, ks + 1, xf,
+                                                      B + ((ks + 1) & 1) * Cfg::BUF_STRIDE, Cfg::TERM_STRIDE);
+                sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 2 < K ? ks + 2 : K - 1, xf);
+                __syncthreads();
             }
         }
         // D fragment of column tile j: lane holds rows 16*mi + 4*(lane/16) + r,
