@@ -13,8 +13,10 @@
 //    slice (entries of a row in ascending global column order);
 //  * MFMA-routed blocks (SpMM) are stored as FP32 in the exact register order
 //    of v_mfma_f32_16x16x32_bf16's A operand (16-row tile m, 32-column k-step
-//    s, lane l: A[16m + l%16][32s + 8(l/16) + j], j = 0..7), so every lane
-//    reads its 8 k-values with two 16-byte loads and splits them in registers.
+//    s, lane l: A[16m + l%16][32s + 8(l/16) + j], j = 0..7).  A fragment
+//    (m, s) is 512 floats: j = 0..3 of lanes 0..63, then j = 4..7 of lanes
+//    0..63, so each of a lane's two 16-byte loads is part of one contiguous
+//    1 KiB wavefront read; the lane splits its 8 k-values in registers.
 //  * column groups never exceed the LDS staging capacity (xcap columns), and
 //    blocks wider than that are split by columns.
 #include "plan.h"
@@ -496,7 +498,8 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                         const int32_t ks = c / 32, kk = c % 32;
                         for (int32_t r = 0; r < hs; ++r) {
                             const int32_t lane = (r % 16) + 16 * (kk / 8);
-                            dst[(((int64_t)(r / 16) * pc.ksteps + ks) * 64 + lane) * 8 + (kk % 8)] = col[r];
+                            dst[((int64_t)(r / 16) * pc.ksteps + ks) * 512 + ((kk % 8) / 4) * 256 + lane * 4 +
+                                (kk % 4)] = col[r];
                         }
                     }
                 }

@@ -14,9 +14,10 @@
 //
 // A (the matrix) is stored by the plan builder as FP32 in MFMA register
 // order: for 16-row tile m and 32-column k-step s, lane l holds A[16m + l%16]
-// [32s + 8(l/16) + j], j = 0..7, as 8 consecutive floats (two 16-byte
-// loads).  Storing FP32 keeps the A stream at 4 bytes per value for either
-// split.  X is staged per k-step: the 32 X rows of the piece are read
+// [32s + 8(l/16) + j], j = 0..7.  The fragment's 512 floats are j = 0..3 of
+// lanes 0..63, then j = 4..7 of lanes 0..63, so the lane's two 16-byte loads
+// each belong to one contiguous 1 KiB wavefront read.  Storing FP32 keeps the
+// A stream at 4 bytes per value for either split.  X is staged per k-step: the 32 X rows of the piece are read
 // (coalesced along N), split, and written to LDS as [n][k] term arrays with a
 // 40-element pitch so the B-operand reads (lane l: n = l%16, k = 8(l/16) ..
 // +7) are conflict-free 16-byte LDS reads.
@@ -35,6 +36,11 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 
 #ifndef SB_SPLIT
 #define SB_SPLIT 3
+#endif
+
+// k-steps of A and X loads in flight ahead of the MFMAs
+#ifndef SB_MFMA_DEPTH
+#define SB_MFMA_DEPTH 2     // 1 to 4
 #endif
 
 __device__ __forceinline__ sb_bf16x8 sb_as_bf16x8(sb_u4 v) { return __builtin_bit_cast(sb_bf16x8, v); }
@@ -131,20 +137,22 @@ __device__ __forceinline__ void sb_store_b(int N, int ncols, int ks, float (*f)[
     }
 }
 
+// The product passes over the term pairs (a_i, x_j) with i + j < SPLIT,
+// smallest terms first: SPLIT = 3 -> a3x1 a2x2 a1x3 a2x1 a1x2 a1x1,
+// SPLIT = 2 -> a2x1 a1x2 a1x1.
+template <int V>
+struct SbInt {
+    static constexpr int value = V;
+};
+
 template <int SPLIT>
-__device__ __forceinline__ sb_f32x4 sb_mfma_terms(const sb_bf16x8 *a, const sb_bf16x8 *b, sb_f32x4 acc)
-{
-    // largest-order terms last so the small ones accumulate first
-    if (SPLIT == 3) {
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
-    return acc;
-}
+struct SbPasses {
+    static constexpr int N = SPLIT == 3 ? 6 : 3;
+    static constexpr int a(int p) { return SPLIT == 3 ? (p == 0 ? 2 : p == 1 ? 1 : p == 2 ? 0 : p == 3 ? 1 : 0)
+                                                      : (p == 0 ? 1 : 0); }
+    static constexpr int b(int p) { return SPLIT == 3 ? (p == 0 ? 0 : p == 1 ? 1 : p == 2 ? 2 : p == 3 ? 0 : p == 4 ? 1 : 0)
+                                                      : (p == 1 ? 1 : 0); }
+};
 
 template <int WG, int NRHS, int SPLIT>
 struct SbMfmaCfg {
@@ -225,68 +233,114 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
             __syncthreads();
             const int K = pc.ksteps;
             const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
-            // pipeline: B of k-step s in LDS buffer s & 1.  The A fragments
-            // of k-step s + 1 are loaded while k-step s computes; the X values
-            // of k-step s + 1 were loaded during k-step s - 1 and are split
-            // into the free buffer after k-step s's MFMAs are issued (the two
-            // overlap).  One barrier per k-step.
-            float xf[Cfg::UPT][8];
-            sb_f4 an[UN][2];
+            // pipeline, DEPTH k-steps deep: A of k-step s sits in register
+            // slot s % DEPTH and X of k-step s in slot s % DEPTH, each loaded
+            // DEPTH k-steps before its use (A is streamed from HBM, so one
+            // k-step of compute cannot cover its latency).  B of k-step s is
+            // in LDS buffer s & 1; X of k-step s + 1 is split into the free
+            // buffer after k-step s's MFMAs are issued.  Every load is
+            // unconditional (indices past the piece clamp to its last k-step)
+            // so the waits on them stay exact; one barrier per k-step.
+            constexpr int D = SB_MFMA_DEPTH;
+            float xf[D][Cfg::UPT][8];
+            sb_f4 ar[D][UN][2];
 #pragma unroll
-            for (int i = 0; i < UN; ++i) {
-                const int off = (umi[i] * K * 64 + lane) * 2;
-                an[i][0] = __builtin_nontemporal_load(&A[off]);
-                an[i][1] = __builtin_nontemporal_load(&A[off + 1]);
+            for (int sl = 0; sl < D; ++sl) {
+                const int kl = sl < K ? sl : K - 1;
+#pragma unroll
+                for (int i = 0; i < UN; ++i) {
+                    const int off = (umi[i] * K + kl) * 128 + lane;
+                    ar[sl][i][0] = __builtin_nontemporal_load(&A[off]);
+                    ar[sl][i][1] = __builtin_nontemporal_load(&A[off + 64]);
+                }
             }
-            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf);
-            sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, 0, xf, B, Cfg::TERM_STRIDE);
-            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, K > 1 ? 1 : 0, xf);
+            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf[0]);
+            sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, 0, xf[0], B, Cfg::TERM_STRIDE);
+#pragma unroll
+            for (int j = 1; j <= D; ++j)
+                sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, j < K ? j : K - 1, xf[j % D]);
             __syncthreads();
-            for (int ks = 0; ks < K; ++ks) {
+            // one k-step with A in register slot sl and the X values of k-step
+            // ks + 1 in slot (sl + 1) % D
+            auto step = [&](auto slc, const int ks) {
+                constexpr int sl = decltype(slc)::value;
                 const uint16_t *Bc = B + (ks & 1) * Cfg::BUF_STRIDE;
-                // A terms of every unit (the registers are then reloaded with
-                // k-step ks + 1), then each B fragment read once from LDS and
-                // applied to all units (units of a wavefront share their
-                // column group)
+                // A terms of every unit (the slot is then reloaded with
+                // k-step ks + D), then each B fragment read once from LDS
+                // and applied to all units (units of a wavefront share
+                // their column group)
                 sb_bf16x8 at[UN][SPLIT];
 #pragma unroll
                 for (int i = 0; i < UN; ++i) {
-                    const float af[8] = {an[i][0].x, an[i][0].y, an[i][0].z, an[i][0].w,
-                                         an[i][1].x, an[i][1].y, an[i][1].z, an[i][1].w};
+                    const float af[8] = {ar[sl][i][0].x, ar[sl][i][0].y, ar[sl][i][0].z, ar[sl][i][0].w,
+                                         ar[sl][i][1].x, ar[sl][i][1].y, ar[sl][i][1].z, ar[sl][i][1].w};
                     sb_split8<SPLIT>(af, at[i]);
                 }
                 {
-                    // unconditional loads (the last k-step reloads itself,
-                    // inactive units read unit 0) keep the control flow
-                    // uniform, so the waits on them stay precise
-                    const int kn = ks + 1 < K ? ks + 1 : ks;
+                    const int kn = ks + D < K ? ks + D : K - 1;
 #pragma unroll
                     for (int i = 0; i < UN; ++i) {
-                        const int off = ((umi[i] * K + kn) * 64 + lane) * 2;
-                        an[i][0] = __builtin_nontemporal_load(&A[off]);
-                        an[i][1] = __builtin_nontemporal_load(&A[off + 1]);
+                        const int off = (umi[i] * K + kn) * 128 + lane;
+                        ar[sl][i][0] = __builtin_nontemporal_load(&A[off]);
+                        ar[sl][i][1] = __builtin_nontemporal_load(&A[off + 64]);
                     }
                 }
+                // two column tiles per step, and the pass loop outside
+                // the unit and column loops, so consecutive MFMAs go to
+                // different accumulators (no back-to-back dependence).
+                // A second column past ntw reads column j0's B and is
+                // never stored.  Active units are a prefix (uact[i]
+                // implies uact[i - 1]), so one branch picks the count.
+                constexpr int JB = Cfg::NTMAX >= 2 ? 2 : 1;
+                auto passes = [&](auto nu) {
+                    constexpr int NU = decltype(nu)::value;
 #pragma unroll
-                for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
-                    if (jj < ntw) {
-                        const int n = (gj0 + jj) * 16 + nb;
-                        sb_bf16x8 bt[SPLIT];
+                    for (int j0 = 0; j0 < Cfg::NTMAX; j0 += JB) {
+                        if (j0 < ntw) {
+                            sb_bf16x8 bt[JB][SPLIT];
 #pragma unroll
-                        for (int s = 0; s < SPLIT; ++s)
-                            bt[s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
+                            for (int c = 0; c < JB; ++c) {
+                                const int n = (gj0 + (j0 + c < ntw ? j0 + c : j0)) * 16 + nb;
 #pragma unroll
-                        for (int i = 0; i < UN; ++i)
-                            if (uact[i]) acc[i][jj] = sb_mfma_terms<SPLIT>(at[i], bt, acc[i][jj]);
+                                for (int s = 0; s < SPLIT; ++s)
+                                    bt[c][s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE +
+                                                                                 n * SB_KP + kb);
+                            }
+                            using Pt = SbPasses<SPLIT>;
+#pragma unroll
+                            for (int p = 0; p < Pt::N; ++p)
+#pragma unroll
+                                for (int i = 0; i < NU; ++i)
+#pragma unroll
+                                    for (int c = 0; c < JB; ++c)
+                                        acc[i][j0 + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                            at[i][Pt::a(p)], bt[c][Pt::b(p)], acc[i][j0 + c], 0, 0, 0);
+                        }
                     }
-                }
+                };
+                if (uact[UN - 1]) passes(SbInt<UN>{});
+                else if (uact[0]) passes(SbInt<1>{});
                 // stage k-step ks + 1 into the free buffer (after the last
-                // k-step this writes zeros nobody reads), then load ks + 2
-                sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, ks + 1, xf,
+                // k-step this writes zeros nobody reads), then load the X
+                // values of k-step ks + 1 + D into the slot it frees
+                constexpr int xs = (sl + 1) % D;
+                sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, ks + 1, xf[xs],
                                                       B + ((ks + 1) & 1) * Cfg::BUF_STRIDE, Cfg::TERM_STRIDE);
-                sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 2 < K ? ks + 2 : K - 1, xf);
+                sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1 + D < K ? ks + 1 + D : K - 1, xf[xs]);
                 __syncthreads();
+            };
+            // groups of D k-steps in a loop with one back-edge and no other
+            // branch around a load, then the remaining k-steps
+            int ks = 0;
+            for (; ks + D <= K; ks += D) {
+                step(SbInt<0>{}, ks);
+                if constexpr (D > 1) step(SbInt<(D > 1 ? 1 : 0)>{}, ks + 1);
+                if constexpr (D > 2) step(SbInt<(D > 2 ? 2 : 0)>{}, ks + 2);
+                if constexpr (D > 3) step(SbInt<(D > 3 ? 3 : 0)>{}, ks + 3);
             }
+            if constexpr (D > 1) if (ks < K) step(SbInt<0>{}, ks);
+            if constexpr (D > 2) if (ks + 1 < K) step(SbInt<(D > 2 ? 1 : 0)>{}, ks + 1);
+            if constexpr (D > 3) if (ks + 2 < K) step(SbInt<(D > 3 ? 2 : 0)>{}, ks + 2);
         }
         // D fragment of column tile j: lane holds rows 16*mi + 4*(lane/16) + r,
         // column 16*j + lane%16

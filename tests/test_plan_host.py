@@ -119,7 +119,7 @@ def reconstruct(plan):
                 for r in range(hs):
                     for c in range(len(cols)):
                         lane = r % 16 + 16 * ((c % 32) // 8)
-                        o = (((r // 16) * ks + c // 32) * 64 + lane) * 8 + c % 8
+                        o = ((r // 16) * ks + c // 32) * 512 + ((c % 8) // 4) * 256 + lane * 4 + c % 4
                         Mb[r0 + r, cols[c]] += dval[off + o]
                         seen[o] = True
                 # every slot not holding a value is zero padding
