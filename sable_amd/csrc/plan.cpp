@@ -398,8 +398,10 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
         S.nblocks_dense += (int64_t)dense.size();
         S.nblocks_sparse += (int64_t)sparse.size();
         S.nblocks_mfma += (int64_t)mfma.size();
-        auto dgroups = group_columns(dense, v, o.xcap);
-        auto sgroups = group_columns(sparse, v, o.xcap);
+        // FP32 SpMM pieces stage their X rows in LDS (SB_SPMM_XS_FLOATS)
+        const int vcap = spmm ? std::max(1, std::min(o.xcap, SB_SPMM_XS_FLOATS / o.n_rhs)) : o.xcap;
+        auto dgroups = group_columns(dense, v, vcap);
+        auto sgroups = group_columns(sparse, v, vcap);
         auto mgroups = group_columns(mfma, v, o.xcap);
 
         // sparse nonzeros per row of the block row (for costing / lanes)

@@ -37,6 +37,10 @@ struct SbTile {
     int32_t y_accum;         // SpMM: 1 = add to Y rows an MFMA tile wrote before
 };
 
+// LDS for the staged X rows of a FP32 SpMM piece (64 KiB): the plan builder
+// gives such a piece at most SB_SPMM_XS_FLOATS / N columns.
+#define SB_SPMM_XS_FLOATS 16384
+
 // SpMM MFMA tiles: a wavefront holds SB_MFMA_UNITS(N) 16-row tiles across
 // all N/16 column tiles (their FP32 accumulators plus the double-buffered A
 // fragments), which bounds a tile at SB_MFMA_MAX_ROWS(N, WG) rows (and 256).

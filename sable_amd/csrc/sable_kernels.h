@@ -416,6 +416,32 @@ __device__ __forceinline__ void sb_expand_xrow(const SbXRange *__restrict__ xr, 
     }
 }
 
+// Stages the X rows of a piece (ncols x N floats, row xrow[c] of X for local
+// column c) into xs4[c * N/4 + v], four float4 loads in flight per thread,
+// coalesced along N.
+template <int WG>
+__device__ __forceinline__ void sb_stage_xrows(const float *__restrict__ X, int N, const int *xrow, int ncols,
+                                               float4 *xs4)
+{
+    const int NV = N >> 2;
+    const int tot = ncols * NV;
+    for (int i0 = 0; i0 < tot; i0 += 4 * WG) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * WG + (int)threadIdx.x;
+            const int c = i < tot ? i / NV : 0;
+            const int q = i < tot ? i - c * NV : 0;
+            v[u] = *reinterpret_cast<const float4 *>(X + (size_t)xrow[c] * N + 4 * q);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * WG + (int)threadIdx.x;
+            if (i < tot) xs4[i] = v[u];
+        }
+    }
+}
+
 struct SbVecMap {
     int rt;       // rows per row group
     int rbeg;     // first tile row of this thread's group
@@ -449,16 +475,18 @@ __device__ __forceinline__ void sb_fma4(float4 &acc, float a, const float4 &x)
 }
 
 // Accumulates one FP32-dense or sparse piece into acc (RTMAX row registers).
-// Dense: columns are processed 4 at a time with every load of the group
-// issued before its FMAs.  Sparse: the piece's entries are staged through
-// LDS (value and X row index) in chunks of SB_SPMM_SPCHUNK, then each thread
-// walks its rows' entries with 4 X-row loads in flight.  spv/spx are LDS
-// scratch of SB_SPMM_SPCHUNK entries.
+// The piece's X rows are in LDS (xs4, staged once per tile), so every X read
+// is an LDS read.  Dense: columns are processed 4 at a time with every load
+// of the group issued before its FMAs.  Sparse: the piece's entries are
+// staged through LDS (value and local column) in chunks of SB_SPMM_SPCHUNK,
+// then each thread walks its rows' entries.  spv/spx are LDS scratch of
+// SB_SPMM_SPCHUNK entries.
 template <int WG, int RTMAX>
 __device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbPiece &pc, int hs, int N,
-                                                  const SbVecMap &m, const int *xrow,
-                                                  const float *__restrict__ X, float4 *acc, float *spv, int *spx)
+                                                  const SbVecMap &m, const float4 *xs4, float4 *acc, float *spv,
+                                                  int *spx)
 {
+    const int NV = N >> 2;
     if (pc.kind == SB_PIECE_DENSE) {
         if (!m.active) return;
         const float *__restrict__ P = a.dval + pc.data_off;
@@ -468,7 +496,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbP
             float av[4][RTMAX];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                xv[u] = *reinterpret_cast<const float4 *>(X + (size_t)xrow[c + u] * N + 4 * m.v);
+                xv[u] = xs4[(c + u) * NV + m.v];
                 const float *__restrict__ col = P + (size_t)(c + u) * hs;
 #pragma unroll
                 for (int i = 0; i < RTMAX; ++i) {
@@ -482,7 +510,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbP
                 for (int i = 0; i < RTMAX; ++i) sb_fma4(acc[i], av[u][i], xv[u]);
         }
         for (; c < pc.ncols; ++c) {
-            const float4 xv = *reinterpret_cast<const float4 *>(X + (size_t)xrow[c] * N + 4 * m.v);
+            const float4 xv = xs4[c * NV + m.v];
             const float *__restrict__ col = P + (size_t)c * hs;
 #pragma unroll
             for (int i = 0; i < RTMAX; ++i) {
@@ -510,7 +538,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbP
         __syncthreads();
         for (int e = threadIdx.x; e < n; e += WG) {
             spv[e] = sv[e0 + e];
-            spx[e] = xrow[si[e0 + e]];
+            spx[e] = si[e0 + e];
         }
         __syncthreads();
         if (!m.active) continue;
@@ -522,18 +550,19 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbP
                 float4 xv[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    xv[u] = *reinterpret_cast<const float4 *>(X + (size_t)spx[e + u] * N + 4 * m.v);
+                    xv[u] = xs4[spx[e + u] * NV + m.v];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) sb_fma4(acc[i], spv[e + u], xv[u]);
             }
             for (; e < hi; ++e)
-                sb_fma4(acc[i], spv[e], *reinterpret_cast<const float4 *>(X + (size_t)spx[e] * N + 4 * m.v));
+                sb_fma4(acc[i], spv[e], xs4[spx[e] * NV + m.v]);
         }
     }
 }
 
 template <int WG, int RTMAX, int NRHS>
-__device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int *xrow, float *spv, int *spx)
+__device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int *xrow, float4 *xs4, float *spv,
+                                                 int *spx)
 {
     sb_clock(a.tile_clock, t, 0);
     const int N = NRHS ? NRHS : a.n_rhs;
@@ -551,7 +580,9 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
         __syncthreads();
         sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
         __syncthreads();
-        sb_spmm_vec_piece<WG, RTMAX>(a, pc, hs, N, m, xrow, X, acc, spv, spx);
+        sb_stage_xrows<WG>(X, N, xrow, pc.ncols, xs4);
+        __syncthreads();
+        sb_spmm_vec_piece<WG, RTMAX>(a, pc, hs, N, m, xs4, acc, spv, spx);
     }
     if (m.active) {
 #pragma unroll
@@ -577,10 +608,15 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
 template <int WG, int RTMAX, int NRHS, int XCAP>
 __device__ __forceinline__ void sb_spmm_f32_body(const SbSpmmArgs &a)
 {
+    // staged X rows of the current piece: pieces have at most
+    // SB_SPMM_XS_FLOATS / N columns (the plan builder cuts them so)
+    constexpr int XS4 = NRHS ? (XCAP < SB_SPMM_XS_FLOATS / NRHS ? XCAP : SB_SPMM_XS_FLOATS / NRHS) * (NRHS / 4)
+                             : SB_SPMM_XS_FLOATS / 4;
     __shared__ int xrow[XCAP + 4];
+    __shared__ float4 xs4[XS4];
     __shared__ float spv[SB_SPMM_SPCHUNK];
     __shared__ int spx[SB_SPMM_SPCHUNK];
     __shared__ int next_tile;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow, spv, spx); });
+                 [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow, xs4, spv, spx); });
 }
