@@ -70,10 +70,11 @@ def fixed_routes(vbr, n_rhs=1, threshold=convert.DEFAULT_DENSITY, mfma=False):
 
 def choose_plan(vbr, x, router=None, n_rhs=1, threshold=convert.DEFAULT_DENSITY, tol=1e-5, reps=20,
                 flush_bytes=1 << 30, **plan_kw):
-    """Builds the fixed-threshold plan and (when a router is available) the
-    learned plan; returns (plan, report).  The learned plan is kept only if
-    its error against the FP64 oracle is within ``tol`` like the fixed plan
-    and its cold median launch time is lower."""
+    """Builds the fixed-threshold plan (for SpMM with the dense blocks on
+    MFMA when N allows it) and, when a router is available, the learned
+    plan; returns (plan, report).  The learned plan is kept only if its
+    error against the FP64 oracle is within ``tol`` like the fixed plan and
+    its cold median launch time is lower."""
     from ..runtime import Plan, DeviceArray
     op = "spmv" if n_rhs == 1 else "spmm"
     xd = DeviceArray.from_numpy(np.asarray(x, dtype=np.float32))
@@ -86,7 +87,8 @@ def choose_plan(vbr, x, router=None, n_rhs=1, threshold=convert.DEFAULT_DENSITY,
         y = yd.numpy()
         return p, t, y
 
-    fixed = fixed_routes(vbr, n_rhs, threshold)
+    # the best fixed policy: threshold routes, dense blocks on MFMA where N allows
+    fixed = fixed_routes(vbr, n_rhs, threshold, mfma=True)
     p_fixed, t_fixed, y_fixed = run(fixed)
     report = {"fixed_ms": t_fixed, "kept": "fixed"}
     if router is None:
