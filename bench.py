@@ -179,9 +179,10 @@ def bench_batch(args, runtime, info):
     uniform 50-800 partitions sized [ceil(n/5P), floor(2n/P)], 1% blocks, the
     100k workload's density mix, seed 5), each with an SpMV and an SpMM with
     N = 16.  Batched = learned routes (MFMA allowed for the SpMM), autotuned
-    tiles, every plan in one group (one persistent launch per operation,
-    both captured in one hipGraph).  Baseline = plain plans (fixed threshold
-    T = 15, FP32 dense path) launched one after another."""
+    tiles sized for the group (no per-plan minimum tile count), every plan in
+    one group (one launch per operation over the concatenated tiles, both
+    captured in one hipGraph).  Baseline = plain plans (fixed threshold
+    T = 15, FP32 dense path, default tiling) launched one after another."""
     from sable_amd import synth
     from sable_amd.convert import threshold_routes
     from sable_amd.router import Router
@@ -202,7 +203,9 @@ def bench_batch(args, runtime, info):
             pp = runtime.Plan.from_vbr(vbr, fixed, op=op, n_rhs=k)
             lr = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
                                       vbr.bpntre, k) if router else fixed)
-            pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k)
+            # built for the group: no per-plan minimum tile count (the group
+            # fills the GPU; small matrices keep block-row-sized tiles)
+            pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1)
             plain.append(pp)
             learned.append(pl)
             xs.append(runtime.DeviceArray.from_numpy(inp))

@@ -167,11 +167,13 @@ int sable_tile_times(SablePlan *plan, float *ns, int32_t ntiles);
 
 /* Batching (no single reference counterpart: replaces running many generated
  * programs, bench_launcher.py:85 / eval.py:30, one process each).  The
- * plans' tiles (each plan with its own x/y) run in persistent launches over
- * one LPT-ordered work list per (op, n_rhs, wg) class of members -- one
- * launch when all members share them, e.g. SpMV then SpMM for mixed
- * groups -- captured once into a hipGraph and replayed by
- * sable_group_launch.  mode as in SablePlanOpts (queue by default). */
+ * plans' tiles (each plan with its own x/y) run in launches over one
+ * LPT-ordered work list per (op, n_rhs, wg) class of members -- one launch
+ * when all members share them, e.g. SpMV then SpMM for mixed groups --
+ * captured once into a hipGraph and replayed by sable_group_launch.  mode
+ * as in SablePlanOpts: SABLE_MODE_GRID (one workgroup per tile) measured
+ * fastest; STATIC and QUEUE launch persistent workgroups.  Members built for
+ * a group need no min_tiles of their own (the group fills the GPU). */
 int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x,
                        float *const *y, int mode, int grid, SableGroup **out);
 int sable_group_launch(SableGroup *g, int sync);

@@ -61,8 +61,8 @@ def test_spmv_group_matches_members(rt, mode):
 
 
 def test_mixed_group_spmv_and_spmm_replays(rt):
-    """SpMV plans and SpMM (N = 16, MFMA dense blocks) plans in one group:
-    two persistent launches in one graph.  Replayed 3 times with eager
+    """SpMV plans and SpMM (N = 16, MFMA dense blocks) plans in one group
+    (default mode): two launches in one graph.  Replayed 3 times with eager
     launches and host synchronisation in between; every replay matches."""
     mats = _mats(5, seed=7)
     plans, xs, ys, own = [], [], [], []
