@@ -168,8 +168,8 @@ struct SbMfmaCfg {
 
 // One MFMA tile (only MFMA pieces; it writes its Y rows).  Work units are
 // (16-row tile mi, column group gi) pairs: ng column groups of nt/ng
-// 16-column tiles, ng the smallest power of two that gives every wavefront
-// a unit when the tile is short.  Wavefront w takes units w, w + NW, ...,
+// 16-column tiles, ng the smallest power of two dividing nt that gives every
+// wavefront a unit when the tile is short (or the largest such).  Wavefront w takes units w, w + NW, ...,
 // UNITS of them per pass (one pass for tiles the host cut to
 // SB_MFMA_MAX_ROWS; more only if a generic kernel meets a taller tile).  Per
 // 32-column k-step the workgroup stages the X rows once (split into bf16
@@ -192,10 +192,11 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     const int mt = (hs + 15) >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    // ng divides NW (both powers of two, ng <= NW), so the units of one
-    // wavefront (w, w + NW, ...) share the column group w % ng
+    // ng divides nt (every column tile is in exactly one group) and NW
+    // (both powers of two, ng <= NW), so the units of one wavefront
+    // (w, w + NW, ...) share the column group w % ng
     int ng = 1;
-    while (ng * 2 <= nt && mt * ng < NW) ng <<= 1;
+    while (ng * 2 <= nt && nt % (ng * 2) == 0 && mt * ng < NW) ng <<= 1;
     const int gj0 = (wave % ng) * (nt / ng);
     const int ntw = nt / ng;
     const int units = mt * ng;
@@ -313,8 +314,9 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
                                 for (int i = 0; i < NU; ++i)
 #pragma unroll
                                     for (int c = 0; c < JB; ++c)
-                                        acc[i][j0 + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                            at[i][Pt::a(p)], bt[c][Pt::b(p)], acc[i][j0 + c], 0, 0, 0);
+                                        if (j0 + c < Cfg::NTMAX)   // odd NTMAX: the last pair has one column
+                                            acc[i][j0 + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                at[i][Pt::a(p)], bt[c][Pt::b(p)], acc[i][j0 + c], 0, 0, 0);
                         }
                     }
                 };

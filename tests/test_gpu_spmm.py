@@ -72,7 +72,7 @@ def test_fp32_widths(rt, n_rhs):
     assert _err(vbr, X, Y)["componentwise"] <= TOL
 
 
-@pytest.mark.parametrize("n_rhs", [32, 64, 256])
+@pytest.mark.parametrize("n_rhs", [32, 48, 64, 80, 256])
 def test_mfma_widths(rt, n_rhs):
     vbr = _c3_like(2000, seed=6)
     X = np.random.default_rng(3).uniform(-1, 1, (vbr.cpntr[-1], n_rhs)).astype(np.float32)
@@ -83,8 +83,9 @@ def test_mfma_widths(rt, n_rhs):
 
 
 def test_three_routes_in_one_block_row(rt):
-    """MFMA, FP32-dense and sparse blocks share block rows: the MFMA and
-    vector partial results meet in the LDS Y tile."""
+    """MFMA, FP32-dense and sparse blocks share block rows: the MFMA tiles
+    write Y in the first launch and the vector tiles add to it in the
+    second."""
     vbr = _c3_like(2500, seed=9)
     rng = np.random.default_rng(4)
     routes = rng.choice([ROUTE_DENSE, ROUTE_SPARSE, ROUTE_DENSE_MFMA], len(vbr.bindx)).astype(np.uint8)
