@@ -106,6 +106,36 @@ def test_mfma_short_and_unit_blocks(rt):
     assert _err(vbr, X, Y)["componentwise"] <= TOL
 
 
+@pytest.mark.parametrize("n_rhs", [16, 48])
+def test_spmm_edge_shapes(rt, n_rhs):
+    """Unit partitions and empty block rows on every route; one 700 x 5000
+    block (wider than the FP32 path's LDS staging and the MFMA staging
+    capacity, taller than an MFMA tile) on every route; an all-zero sparse
+    block gives Y = 0."""
+    from sable_amd.vbr import VBR
+    vbr, _ = synth.vbr_matrix_gen_seeded(300, 280, 60, 60, 200, seed=19)
+    assert np.any(np.asarray(vbr.bpntrb) == -1)
+    X = np.random.default_rng(8).uniform(-1, 1, (280, n_rhs)).astype(np.float32)
+    for route in (ROUTE_DENSE, ROUTE_SPARSE, ROUTE_DENSE_MFMA):
+        print(f"case: spmm unit partitions n={n_rhs} route={route}", flush=True)
+        Y = rt.Plan.from_vbr(vbr, np.full(len(vbr.bindx), route, np.uint8), op="spmm", n_rhs=n_rhs).spmm(X)
+        assert _err(vbr, X, Y)["componentwise"] <= TOL
+    rng = np.random.default_rng(2)
+    h, w = 700, 5000
+    val = rng.standard_normal(h * w).astype(np.float32)
+    val[rng.random(h * w) < 0.7] = 0
+    blk = dict(val=val, indx=[0, h * w], bindx=[0], rpntr=[0, h], cpntr=[0, w], bpntrb=[0], bpntre=[1])
+    Xw = rng.uniform(-1, 1, (w, n_rhs)).astype(np.float32)
+    V = VBR(**blk)
+    for route in (ROUTE_DENSE, ROUTE_SPARSE, ROUTE_DENSE_MFMA):
+        print(f"case: spmm 700x5000 n={n_rhs} route={route}", flush=True)
+        Y = rt.Plan(**blk, routes=[route], op="spmm", n_rhs=n_rhs).spmm(Xw)
+        assert _err(V, Xw, Y)["componentwise"] <= TOL
+    zero = dict(blk, val=np.zeros(h * w, np.float32))
+    Y = rt.Plan(**zero, routes=[ROUTE_SPARSE], op="spmm", n_rhs=n_rhs).spmm(Xw)
+    assert np.all(Y == 0)
+
+
 def test_matches_reference_spmm_program(rt):
     """Against the restated reference SpMM program (src/codegen.py:827-937,
     N as a parameter, density-routed inline)."""
