@@ -211,6 +211,21 @@ int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int r
  * counts read + write bytes). */
 int sable_bandwidth_probe(int64_t bytes, int reps, float *read_gbs, float *copy_gbs);
 
+/* Block partitioner, one axis (src/autopartition.py:34-68 cut_indices2 with
+ * similarity2 :73-74, similarity2_numba :76-94 or similarity1 :70-71; the
+ * reference densifies every line on the host).  The lines are the columns of
+ * a CSC matrix given in host memory: indptr (nlines + 1 entries, indptr[0] =
+ * 0, nondecreasing), indices ascending and unique within a line and below
+ * line_len, FP64 data.  cut (nlines + 1 bytes) receives 1 where the
+ * reference's scan cuts (always at 0 and nlines); sim (nlines - 1 doubles,
+ * may be null) the adjacent-pair similarities, each dot product summed in
+ * ascending index order without contraction; kernel_ms (may be null) the
+ * time of the two kernels.  kind: 0 similarity2, 1 similarity2_numba,
+ * 2 similarity1.  Rows are the same call on the transpose. */
+int sable_axis_cuts(const int64_t *indptr, const int32_t *indices, const double *data, int64_t nlines,
+                    int64_t line_len, int kind, double threshold, uint8_t *cut, double *sim,
+                    float *kernel_ms);
+
 /* Device memory helpers for callers without another allocator. */
 int sable_malloc(void **ptr, int64_t bytes);
 int sable_free(void *ptr);

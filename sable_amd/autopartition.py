@@ -15,8 +15,10 @@ names (numba is not needed).
 
 Instead of densifying two lines per step (the reference's per-line
 ``toarray()``), the three dot products of every adjacent pair are computed at
-once with sparse elementwise products, and only the final scan (which is
-inherently sequential because of the empty-run rule) loops in Python.  A
+once with sparse elementwise products (each summed in ascending index
+order), and only the final scan loops in Python.  ``device=True`` runs both
+on the GPU (``sable_axis_cuts``: one thread per adjacent pair, and the scan
+in its closed form per position), with the same result bit for bit.  A
 user-supplied similarity function falls back to the per-pair loop.
 """
 from __future__ import annotations
@@ -62,17 +64,30 @@ def similarity1(a, b):
     return (a.dot(b) + a[1:].dot(b[:-1]) + a[:-1].dot(b[1:])) / (3 * max(np.count_nonzero(a), np.count_nonzero(b)))
 
 
+def _colsum(P):
+    """Column sums of a sparse matrix, each summed in ascending row order (one
+    rounding per addition: np.bincount accumulates in index order), the
+    order the GPU partitioner uses too."""
+    P = scipy.sparse.csc_matrix(P)
+    P.sort_indices()
+    cols = np.repeat(np.arange(P.shape[1]), np.diff(P.indptr))
+    return np.bincount(cols, weights=P.data, minlength=P.shape[1]).astype(np.float64)
+
+
 def _pair_similarities(M, kind):
-    """Similarity of every adjacent column pair (i, i+1) of the CSC matrix M."""
+    """Similarity of every adjacent column pair (i, i+1) of the CSC matrix M.
+    Each dot product is the sequential ascending-row sum of the entry
+    products (exact for integer values, the reference's tests/Franz8.mtx
+    included)."""
     n = M.shape[1]
     if n < 2:
         return np.zeros(0)
     L = M[:, :-1]
     R = M[:, 1:]
-    d0 = np.asarray(L.multiply(R).sum(axis=0)).ravel()
+    d0 = _colsum(L.multiply(R))
     if M.shape[0] > 1:
-        d1 = np.asarray(L[1:, :].multiply(R[:-1, :]).sum(axis=0)).ravel()
-        d2 = np.asarray(L[:-1, :].multiply(R[1:, :]).sum(axis=0)).ravel()
+        d1 = _colsum(L[1:, :].multiply(R[:-1, :]))
+        d2 = _colsum(L[:-1, :].multiply(R[1:, :]))
         num = np.maximum(d0, np.maximum(d1, d2))
     else:
         num = d0
@@ -128,11 +143,40 @@ def _axis_cuts(M, threshold, similarity):
     return _scan(nnz, sim, threshold)
 
 
-def cut_indices2(A, cut_threshold, similarity):
-    """src/autopartition.py:34-68.  Returns (col_indices, row_indices)."""
+_GPU_KIND = {similarity2: 0, similarity2_numba: 1, similarity1: 2}
+
+
+def _axis_cuts_gpu(M, threshold, similarity, return_sim=False):
+    """Cuts along the columns of M on the GPU (sable_axis_cuts): the same
+    similarities as the host path, bit for bit, and the same scan."""
+    from . import runtime
+    from ._lib import lib, check
+    kind = _GPU_KIND.get(similarity)
+    if kind is None:
+        raise ValueError("the GPU partitioner takes similarity2, similarity2_numba or similarity1")
+    M = scipy.sparse.csc_matrix(M).astype(np.float64)
+    M.sum_duplicates()
+    M.sort_indices()
+    n = M.shape[1]
+    ip = np.ascontiguousarray(M.indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(M.indices, dtype=np.int32)
+    v = np.ascontiguousarray(M.data, dtype=np.float64)
+    cut = np.zeros(n + 1, dtype=np.uint8)
+    sim = np.zeros(max(n - 1, 1), dtype=np.float64)
+    runtime.init()
+    check(lib.sable_axis_cuts(ip.ctypes.data, ix.ctypes.data, v.ctypes.data, n, M.shape[0], kind,
+                              float(threshold), cut.ctypes.data, sim.ctypes.data, None))
+    cuts = np.flatnonzero(cut).tolist()
+    return (cuts, sim[:max(n - 1, 0)]) if return_sim else cuts
+
+
+def cut_indices2(A, cut_threshold, similarity, device=False):
+    """src/autopartition.py:34-68.  Returns (col_indices, row_indices).
+    device=True computes both axes on the GPU (same result)."""
     A = scipy.sparse.csc_matrix(A)
-    cols = _axis_cuts(A, cut_threshold, similarity)
-    rows = _axis_cuts(A.T.tocsc(), cut_threshold, similarity)
+    axis = _axis_cuts_gpu if device else _axis_cuts
+    cols = axis(A, cut_threshold, similarity)
+    rows = axis(A.T.tocsc(), cut_threshold, similarity)
     return cols, rows
 
 

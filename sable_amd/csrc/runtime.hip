@@ -30,6 +30,7 @@
 #include "sable_kernels.h"
 #include "sable_mfma.h"
 #include "sable_mlp.h"
+#include "sable_partition.h"
 
 // ---------------------------------------------------------------------------
 // ahead-of-time generic instances (fallback when hipRTC is unavailable)
@@ -1247,6 +1248,84 @@ int sable_bandwidth_probe(int64_t bytes, int reps, float *read_gbs, float *copy_
     hipFree(out);
     if (read_gbs) *read_gbs = best_r;
     if (copy_gbs) *copy_gbs = best_c;
+    return SABLE_OK;
+}
+
+// ---- block partitioner: adjacent-line similarity and cut flags
+
+namespace {
+
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs()
+    {
+        for (void *q : p) hipFree(q);
+    }
+    int alloc(void **out, int64_t bytes)
+    {
+        void *q = nullptr;
+        HIPOK(hipMalloc(&q, std::max<int64_t>(bytes, 1)));
+        p.push_back(q);
+        *out = q;
+        return SABLE_OK;
+    }
+};
+
+}  // namespace
+
+int sable_axis_cuts(const int64_t *indptr, const int32_t *indices, const double *data, int64_t nlines,
+                    int64_t line_len, int kind, double threshold, uint8_t *cut, double *sim, float *kernel_ms)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (nlines < 0 || line_len < 0 || !indptr || !cut) return fail(SABLE_ERR_INVALID, "bad axis_cuts arguments");
+    if (kind < SB_SIM_S2 || kind > SB_SIM_S1) return fail(SABLE_ERR_INVALID, "unknown similarity kind");
+    // the kernels read only inside [indptr[i], indptr[i+1]): check the bounds
+    if (indptr[0] != 0) return fail(SABLE_ERR_INVALID, "indptr[0] must be 0");
+    for (int64_t i = 0; i < nlines; ++i)
+        if (indptr[i + 1] < indptr[i]) return fail(SABLE_ERR_INVALID, "indptr must be nondecreasing");
+    const int64_t nnz = indptr[nlines];
+    if (nnz > 0 && (!indices || !data)) return fail(SABLE_ERR_INVALID, "null indices/data");
+    if (kernel_ms) *kernel_ms = 0.f;
+    if (nlines == 0) {
+        cut[0] = 1;
+        return SABLE_OK;
+    }
+    DevBufs b;
+    int64_t *d_ip = nullptr;
+    int32_t *d_ix = nullptr;
+    double *d_v = nullptr, *d_sim = nullptr;
+    uint8_t *d_cut = nullptr;
+    CHECKRC(b.alloc((void **)&d_ip, (nlines + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_ix, nnz * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_v, nnz * sizeof(double)));
+    CHECKRC(b.alloc((void **)&d_sim, nlines * sizeof(double)));
+    CHECKRC(b.alloc((void **)&d_cut, nlines + 1));
+    HIPOK(hipMemcpy(d_ip, indptr, (nlines + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        HIPOK(hipMemcpy(d_ix, indices, nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPOK(hipMemcpy(d_v, data, nnz * sizeof(double), hipMemcpyHostToDevice));
+    }
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    hipEventRecord(e0, nullptr);
+    if (nlines > 1)
+        hipLaunchKernelGGL(sb_adjacent_sim, dim3((unsigned)((nlines - 1 + 255) / 256)), dim3(256), 0, nullptr, d_ip,
+                           d_ix, d_v, nlines, line_len, kind, d_sim);
+    hipLaunchKernelGGL(sb_cut_flags, dim3((unsigned)((nlines + 1 + 255) / 256)), dim3(256), 0, nullptr, d_ip, d_sim,
+                       nlines, threshold, d_cut);
+    hipEventRecord(e1, nullptr);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPOK(le);
+    HIPOK(se);
+    if (kernel_ms) *kernel_ms = ms;
+    HIPOK(hipMemcpy(cut, d_cut, nlines + 1, hipMemcpyDeviceToHost));
+    if (sim && nlines > 1) HIPOK(hipMemcpy(sim, d_sim, (nlines - 1) * sizeof(double), hipMemcpyDeviceToHost));
     return SABLE_OK;
 }
 
