@@ -17,7 +17,7 @@ per launch / measured kernel time vs 8 TB/s HBM), both plans' figures, and
 the reference CPU path (the restated generated-C program, FP64 like the
 reference) timed on this host's cores on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5] [--warm]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5|partition] [--warm]
                     [--routing plain|learned|both] [--no-cpu]
 """
 from __future__ import annotations
@@ -269,6 +269,85 @@ def bench_batch(args, runtime, info):
     print(json.dumps(line), flush=True)
 
 
+def bench_partition(args, runtime, info):
+    """Block partitioner workload (src/autopartition.py:34-68, cut_indices2
+    with similarity2 at threshold 0.2, both axes) on the 100,000 x 100,000
+    SpMV matrix's nonzero pattern (values 1.0, FP64).  The
+    GPU time is the two kernels of sable_axis_cuts per axis (inputs copied
+    to the device before the events); the cuts are checked against the host
+    partitioner bit for bit.  CPU baseline: the reference algorithm (the
+    oracle's restatement: every line densified, numpy dot products) on a
+    square corner, scaled by line length to lines per second."""
+    import ctypes as C
+    from sable_amd import synth, autopartition as AP
+    from sable_amd._lib import lib, check
+    from sable_amd.convert import vbr_to_csc
+    vbr, _, _ = synth.workload_c2()
+    A = vbr_to_csc(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    del vbr
+    A.data[:] = 1.0          # its pattern: columns of a dense block are alike, the cuts find the blocks
+    A.sum_duplicates()
+    A.sort_indices()
+    AT = A.T.tocsc()
+    AT.sort_indices()
+    log(f"partition: {A.shape[0]}x{A.shape[1]} matrix, {A.nnz} nonzeros")
+    axes = []
+    bytes_alg = 0
+    for M in (A, AT):
+        n = M.shape[1]
+        arrs = (np.ascontiguousarray(M.indptr, dtype=np.int64), np.ascontiguousarray(M.indices, dtype=np.int32),
+                np.ascontiguousarray(M.data, dtype=np.float64))
+        axes.append((M, n, arrs, np.zeros(n + 1, np.uint8)))
+        # read indptr, indices and values once; write the similarities and cut flags
+        bytes_alg += 8 * (n + 1) + 12 * M.nnz + 8 * (n - 1) + (n + 1)
+    times = []
+    for rep in range(args.warmup + args.steps):
+        total = 0.0
+        for M, n, (ip, ix, v), cut in axes:
+            ms = C.c_float(0.0)
+            check(lib.sable_axis_cuts(ip.ctypes.data, ix.ctypes.data, v.ctypes.data, n, M.shape[0], 0, 0.2,
+                                      cut.ctypes.data, None, C.byref(ms)))
+            total += ms.value
+        if rep >= args.warmup:
+            times.append(total)
+    gpu_cuts = [np.flatnonzero(cut).tolist() for _, _, _, cut in axes]
+    t0 = time.time()
+    host_cuts = [AP._axis_cuts(M, 0.2, AP.similarity2) for M, _, _, _ in axes]
+    host_s = time.time() - t0
+    assert gpu_cuts == host_cuts, "GPU and host partitions differ"
+    ms = float(np.mean(times))
+    lines = A.shape[0] + A.shape[1]
+    out = {
+        "metric": "block partitioner (cut_indices2, similarity2): adjacent-line similarity scan, both axes",
+        "value": bytes_alg / (ms * 1e-3) / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (values), int32 (indices)", "data": "synthetic",
+        "config": {"workload": "cut_indices2(A, 0.2, similarity2) on the pattern of the 100000x100000 SpMV "
+                               f"matrix ({A.nnz} nonzeros), both axes"},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": bytes_alg / (ms * 1e-3) / 1e9 / 8000.0, "traffic": None},
+        "lines_per_s": lines / (ms * 1e-3), "cuts": [len(c) for c in gpu_cuts], "bytes_alg": bytes_alg,
+        "host_partitioner_s": host_s, "matches_host": True, "cpu_baseline": None, "device": info,
+    }
+    if not args.no_cpu:
+        # the reference algorithm densifies every line (cost linear in the line
+        # length): time it on a square corner and scale by line elements
+        from oracle import vbr_oracle
+        m = 10000
+        S = A[:m, :m].tocsc()
+        t0 = time.time()
+        vbr_oracle.cut_indices2_dense_lines(S, 0.2)
+        dt = time.time() - t0
+        rate = 2 * m * m / dt
+        full_s = 2.0 * A.shape[0] * A.shape[1] / rate
+        out["cpu_baseline"] = {"value": lines / full_s, "unit": "lines/s", "cores": 1, "kind": "port",
+                               "sample": f"the reference algorithm (oracle restatement: every line densified, "
+                                         f"numpy dots) on the {m}x{m} corner of the matrix ({2 * m} lines of "
+                                         f"{m}, {dt:.1f} s), scaled by line length to the full matrix",
+                               "est_full_matrix_s": full_s}
+    print(json.dumps(out), flush=True)
+
+
 def spmv_options(args):
     """Plan options: the autotuned table's winners unless given on the command line."""
     from sable_amd.autotune import tuned_options
@@ -284,7 +363,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "partition"])
     ap.add_argument("--count", type=int, default=256, help="c5: number of matrices")
     ap.add_argument("--warm", action="store_true", help="no cache flush between launches")
     ap.add_argument("--routing", default="both", choices=["plain", "learned", "both"],
@@ -318,6 +397,8 @@ def main():
         desc = "SpMV FP32, 1000x1000 VBR, 20x20 random partitions, 40 dense blocks, seed 1"
     elif args.workload == "c5":
         return bench_batch(args, runtime, info)
+    elif args.workload == "partition":
+        return bench_partition(args, runtime, info)
     else:
         return bench_spmm(args, runtime, info)
     log(f"workload generated in {time.time() - t0:.1f}s: {len(vbr.val)} stored values, {len(vbr.bindx)} blocks")

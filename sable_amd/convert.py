@@ -244,6 +244,22 @@ def vbr_to_dense(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre):
     return M
 
 
+def vbr_to_csc(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre):
+    """The VBR matrix as a scipy CSC matrix of its nonzero values, without a
+    dense expansion: stored value p of block k sits at row r0 + (p mod h),
+    column c0 + (p div h) (column-major blocks, src/vbr.py:5-14)."""
+    _, _, r0, h, c0, _ = block_table(rpntr, cpntr, bindx, bpntrb, bpntre)
+    val = np.asarray(val)
+    ix = np.asarray(indx, dtype=np.int64)
+    nz = np.flatnonzero(val)
+    k = np.searchsorted(ix, nz, side="right") - 1
+    off = nz - ix[k]
+    rows = np.asarray(r0, dtype=np.int64)[k] + off % np.asarray(h, dtype=np.int64)[k]
+    cols = np.asarray(c0, dtype=np.int64)[k] + off // np.asarray(h, dtype=np.int64)[k]
+    shape = (int(np.asarray(rpntr)[-1]), int(np.asarray(cpntr)[-1]))
+    return scipy.sparse.csc_matrix((val[nz].astype(np.float64), (rows, cols)), shape=shape)
+
+
 def vbr_to_mtx(filename, dir_name, vbr_dir):
     """utils/mtx_matrices_gen.py:29-49: read ``{vbr_dir}/{filename}`` (.vbr),
     write ``{dir_name}/{stem}.mtx`` and return the dense matrix."""

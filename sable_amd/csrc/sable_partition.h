@@ -4,8 +4,8 @@
 //
 // One axis at a time: the lines are the columns of a CSC matrix (indptr,
 // indices ascending and unique within a line, FP64 values); rows are done on
-// the transpose.  For every adjacent pair (i, i+1) one thread walks the two
-// sorted index lists three times and forms
+// the transpose.  For every adjacent pair (i, i+1) three lanes each walk the
+// two sorted index lists once and form
 //   d0 = sum a[r] b[r],  d1 = sum a[r+1] b[r],  d2 = sum a[r] b[r+1]
 // in ascending r, each product and sum rounded on its own (contraction is
 // switched off: HIP otherwise fuses a * b + c into one FMA),
@@ -55,24 +55,39 @@ __device__ __forceinline__ double sb_shift_dot(const int32_t *__restrict__ ix, c
     return acc;
 }
 
+// Four lanes per adjacent pair: lanes 0-2 walk the merges for d0, d1, d2,
+// lane 3 counts the nonzero values; lane 0 gathers them (4-lane shuffles)
+// and writes the similarity.  Launch 4 * (nlines - 1) threads.
 __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict__ ip,
                                                        const int32_t *__restrict__ ix,
                                                        const double *__restrict__ v, int64_t nlines,
                                                        int64_t line_len, int kind, double *__restrict__ sim)
 {
 #pragma clang fp contract(off)
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + 1 >= nlines) return;
-    const int64_t a0 = ip[i], a1 = ip[i + 1], b1 = ip[i + 2];
-    int64_t ca = 0, cb = 0;
-    for (int64_t k = a0; k < a1; ++k) ca += v[k] != 0.0;
-    for (int64_t k = a1; k < b1; ++k) cb += v[k] != 0.0;
-    const double d0 = sb_shift_dot(ix, v, a0, a1, a1, b1, 0);
-    const double den = (double)(ca > cb ? ca : cb);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t >> 2;
+    const int role = (int)(t & 3);
+    const bool live = i + 1 < nlines;
+    double d = 0.0, nz = 0.0;
+    if (live) {
+        const int64_t a0 = ip[i], a1 = ip[i + 1], b1 = ip[i + 2];
+        if (role == 3) {
+            int64_t ca = 0, cb = 0;
+            for (int64_t k = a0; k < a1; ++k) ca += v[k] != 0.0;
+            for (int64_t k = a1; k < b1; ++k) cb += v[k] != 0.0;
+            d = (double)ca;
+            nz = (double)cb;
+        } else if (role == 0 || line_len > 1) {
+            d = sb_shift_dot(ix, v, a0, a1, a1, b1, role == 0 ? 0 : role == 1 ? 1 : -1);
+        }
+    }
+    // the 4 lanes of a pair are consecutive lanes of one wavefront
+    const double d0 = __shfl(d, 0, 4), d1 = __shfl(d, 1, 4), d2 = __shfl(d, 2, 4);
+    const double ca = __shfl(d, 3, 4), cb = __shfl(nz, 3, 4);
+    if (!live || role != 0) return;
+    const double den = ca > cb ? ca : cb;
     double out;
     if (line_len > 1) {
-        const double d1 = sb_shift_dot(ix, v, a0, a1, a1, b1, 1);     // a[r+1] b[r]
-        const double d2 = sb_shift_dot(ix, v, a0, a1, a1, b1, -1);    // a[r] b[r+1]
         if (kind == SB_SIM_S1) {
             out = ((d0 + d1) + d2) / (3.0 * den);
         } else {
@@ -82,7 +97,7 @@ __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict
     } else {
         out = kind == SB_SIM_S1 ? d0 / (3.0 * den) : d0 / den;
     }
-    if (kind == SB_SIM_NUMBA && (ca == 0 || cb == 0)) out = 0.0;
+    if (kind == SB_SIM_NUMBA && (ca == 0.0 || cb == 0.0)) out = 0.0;
     sim[i] = out;
 }
 

@@ -147,6 +147,21 @@ def test_vbr_to_dense_matches_oracle_and_mtx(kat, golden, tmp_path):
     assert open(tmp_path / "example.mtx").read() == open(os.path.join(golden, "example-canon.mtx")).read()
 
 
+def test_vbr_to_csc_matches_oracle_dense(kat):
+    """The sparse expansion equals the oracle's dense expansion (the
+    reference docstring KAT and random matrices with empty block rows and
+    explicit zeros inside blocks)."""
+    ex = kat["example_vbr"]
+    args = (ex["val"], ex["indx"], ex["bindx"], kat["rpntr"], kat["cpntr"], ex["bpntrb"], ex["bpntre"])
+    assert np.array_equal(C.vbr_to_csc(*args).toarray(), np.array(kat["dense_docstring"], dtype=float))
+    for seed in range(4):
+        vbr, _, _ = synth.mixed_vbr(300, 12, 5, 40, 0.2, 0.5, (0.8, 1.0), (0.01, 0.3), seed)
+        a = (vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+        D = np.asarray(O.vbr_to_dense(*a), dtype=np.float64)
+        S = C.vbr_to_csc(*a)
+        assert np.array_equal(S.toarray(), D) and S.nnz == np.count_nonzero(D)
+
+
 def test_validate_rejects_malformed():
     good = dict(val=[1.0] * 4, indx=[0, 4], bindx=[0], rpntr=[0, 2], cpntr=[0, 2], bpntrb=[0], bpntre=[1])
     validate_vbr(**good)
