@@ -1310,7 +1310,7 @@ int sable_axis_cuts(const int64_t *indptr, const int32_t *indices, const double 
     HIPOK(hipEventCreate(&e1));
     hipEventRecord(e0, nullptr);
     if (nlines > 1)
-        hipLaunchKernelGGL(sb_adjacent_sim, dim3((unsigned)((4 * (nlines - 1) + 255) / 256)), dim3(256), 0, nullptr, d_ip,
+        hipLaunchKernelGGL(sb_adjacent_sim, dim3((unsigned)((nlines - 1 + 3) / 4)), dim3(256), 0, nullptr, d_ip,
                            d_ix, d_v, nlines, line_len, kind, d_sim);
     hipLaunchKernelGGL(sb_cut_flags, dim3((unsigned)((nlines + 1 + 255) / 256)), dim3(256), 0, nullptr, d_ip, d_sim,
                        nlines, threshold, d_cut);

@@ -4,8 +4,7 @@
 //
 // One axis at a time: the lines are the columns of a CSC matrix (indptr,
 // indices ascending and unique within a line, FP64 values); rows are done on
-// the transpose.  For every adjacent pair (i, i+1) three lanes each walk the
-// two sorted index lists once and form
+// the transpose.  For every adjacent pair (i, i+1) one wavefront forms
 //   d0 = sum a[r] b[r],  d1 = sum a[r+1] b[r],  d2 = sum a[r] b[r+1]
 // in ascending r, each product and sum rounded on its own (contraction is
 // switched off: HIP otherwise fuses a * b + c into one FMA),
@@ -55,37 +54,71 @@ __device__ __forceinline__ double sb_shift_dot(const int32_t *__restrict__ ix, c
     return acc;
 }
 
-// Four lanes per adjacent pair: lanes 0-2 walk the merges for d0, d1, d2,
-// lane 3 counts the nonzero values; lane 0 gathers them (4-lane shuffles)
-// and writes the similarity.  Launch 4 * (nlines - 1) threads.
+// One wavefront per adjacent pair (four pairs per 256-thread workgroup).
+// Per chunk of 64 entries of line a, lane l takes entry k = chunk + l (a
+// coalesced read), binary-searches row ra - 1 in line b and checks the next
+// three entries, forming its products for d0 (rb = ra), d1 (rb = ra - 1) and
+// d2 (rb = ra + 1).  Lanes 0-2 then add the chunk's products of d0 / d1 / d2
+// from LDS in entry order, so every sum is the sequential ascending-row sum
+// of the scalar merge.  Nonzero counts are wavefront sums of 0/1 flags.
 __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict__ ip,
                                                        const int32_t *__restrict__ ix,
                                                        const double *__restrict__ v, int64_t nlines,
                                                        int64_t line_len, int kind, double *__restrict__ sim)
 {
 #pragma clang fp contract(off)
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t i = t >> 2;
-    const int role = (int)(t & 3);
-    const bool live = i + 1 < nlines;
-    double d = 0.0, nz = 0.0;
-    if (live) {
-        const int64_t a0 = ip[i], a1 = ip[i + 1], b1 = ip[i + 2];
-        if (role == 3) {
-            int64_t ca = 0, cb = 0;
-            for (int64_t k = a0; k < a1; ++k) ca += v[k] != 0.0;
-            for (int64_t k = a1; k < b1; ++k) cb += v[k] != 0.0;
-            d = (double)ca;
-            nz = (double)cb;
-        } else if (role == 0 || line_len > 1) {
-            d = sb_shift_dot(ix, v, a0, a1, a1, b1, role == 0 ? 0 : role == 1 ? 1 : -1);
-        }
+    __shared__ double prod[4][3][64];
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int64_t i = (int64_t)blockIdx.x * 4 + wave;
+    if (i + 1 >= nlines) return;                      // the whole wavefront
+    const int64_t a0 = ip[i], a1 = ip[i + 1], b0 = a1, b1 = ip[i + 2];
+    int ca = 0, cb = 0;
+    for (int64_t k = a0 + lane; k < a1; k += 64) ca += v[k] != 0.0;
+    for (int64_t k = b0 + lane; k < b1; k += 64) cb += v[k] != 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ca += __shfl_xor(ca, o);
+        cb += __shfl_xor(cb, o);
     }
-    // the 4 lanes of a pair are consecutive lanes of one wavefront
-    const double d0 = __shfl(d, 0, 4), d1 = __shfl(d, 1, 4), d2 = __shfl(d, 2, 4);
-    const double ca = __shfl(d, 3, 4), cb = __shfl(nz, 3, 4);
-    if (!live || role != 0) return;
-    const double den = ca > cb ? ca : cb;
+    double acc = 0.0;                                 // lane 0: d0, lane 1: d1, lane 2: d2
+    for (int64_t c0 = a0; c0 < a1; c0 += 64) {
+        const int64_t k = c0 + lane;
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+        if (k < a1) {
+            const int32_t ra = ix[k];
+            const double va = v[k];
+            int64_t lo = b0, hi = b1;
+            while (lo < hi) {                          // first entry of b with row >= ra - 1
+                const int64_t mid = (lo + hi) >> 1;
+                if (ix[mid] < ra - 1) lo = mid + 1;
+                else hi = mid;
+            }
+            for (int q = 0; q < 3 && lo + q < b1; ++q) {
+                const int32_t rb = ix[lo + q];
+                if (rb > ra + 1) break;
+                const double pr = va * v[lo + q];
+                if (rb == ra) p0 = pr;
+                else if (rb == ra - 1) p1 = pr;
+                else p2 = pr;
+            }
+        }
+        prod[wave][0][lane] = p0;
+        prod[wave][1][lane] = p1;
+        prod[wave][2][lane] = p2;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 3) {
+            const int n = a1 - c0 < 64 ? (int)(a1 - c0) : 64;
+            for (int j = 0; j < n; ++j) acc = acc + prod[wave][lane][j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const double d0 = __shfl(acc, 0), d1 = __shfl(acc, 1), d2 = __shfl(acc, 2);
+    if (lane != 0) return;
+    const double den = (double)(ca > cb ? ca : cb);
     double out;
     if (line_len > 1) {
         if (kind == SB_SIM_S1) {
@@ -97,7 +130,7 @@ __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict
     } else {
         out = kind == SB_SIM_S1 ? d0 / (3.0 * den) : d0 / den;
     }
-    if (kind == SB_SIM_NUMBA && (ca == 0.0 || cb == 0.0)) out = 0.0;
+    if (kind == SB_SIM_NUMBA && (ca == 0 || cb == 0)) out = 0.0;
     sim[i] = out;
 }
 
