@@ -53,6 +53,7 @@ extern "C" {
 
 typedef struct SablePlan SablePlan;
 typedef struct SableGroup SableGroup;
+typedef struct SableInspect SableInspect;
 
 /* The VBR arrays of src/vbr.py:7-14 (validated: see sable_plan_create). */
 typedef struct {
@@ -225,6 +226,25 @@ int sable_bandwidth_probe(int64_t bytes, int reps, float *read_gbs, float *copy_
 int sable_axis_cuts(const int64_t *indptr, const int32_t *indices, const double *data, int64_t nlines,
                     int64_t line_len, int kind, double threshold, uint8_t *cut, double *sim,
                     float *kernel_ms);
+
+/* Block materialisation on the device (utils/convert_real_to_vbr.py:98-199,
+ * convert_sparse_to_vbr: the reference slices every block on the host).
+ * Input: a CSC matrix in host memory (indptr of ncols + 1, row indices below
+ * nrows, duplicates summed, FP64 data) and the partitions rpntr (nbr + 1
+ * entries, 0 .. nrows, strictly increasing) and cpntr (nbc + 1).  A block is
+ * kept when it holds a stored entry; kept blocks are numbered block-row
+ * major and stored densely column-major, as the reference does.  create runs
+ * the device pipeline and returns the sizes (nblocks kept blocks, nval
+ * values) and the device time after the upload; fetch copies the VBR arrays
+ * out (bpntrb / bpntre: -1 for a block row without blocks; block_nnz: the
+ * nonzero values per block, the routing count of
+ * utils/convert_real_to_vbr.py:41-47); any output may be null. */
+int sable_inspect_create(const int64_t *indptr, const int32_t *indices, const double *data, int64_t nrows,
+                         int64_t ncols, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
+                         SableInspect **out, int64_t *nblocks, int64_t *nval, float *device_ms);
+int sable_inspect_fetch(SableInspect *h, double *val, int64_t *indx, int64_t *bindx, int64_t *bpntrb,
+                        int64_t *bpntre, int32_t *block_nnz);
+int sable_inspect_destroy(SableInspect *h);
 
 /* Device memory helpers for callers without another allocator. */
 int sable_malloc(void **ptr, int64_t bytes);

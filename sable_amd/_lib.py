@@ -110,6 +110,10 @@ _SIGS = {
     "sable_adam_step": (I32, [P, P, P, P, I64, C.c_float, C.c_float, C.c_float, C.c_float, I32]),
     "sable_bandwidth_probe": (I32, [I64, I32, F32P, F32P]),
     "sable_axis_cuts": (I32, [P, P, P, I64, I64, I32, C.c_double, P, P, F32P]),
+    "sable_inspect_create": (I32, [P, P, P, I64, I64, P, I32, P, I32, C.POINTER(P), C.POINTER(I64),
+                                   C.POINTER(I64), F32P]),
+    "sable_inspect_fetch": (I32, [P, P, P, P, P, P, P]),
+    "sable_inspect_destroy": (I32, [P]),
     "sable_malloc": (I32, [C.POINTER(P), I64]),
     "sable_free": (I32, [P]),
     "sable_memcpy_h2d": (I32, [P, P, I64]),

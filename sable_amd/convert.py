@@ -181,8 +181,40 @@ def decompress_vbrc(val2, indx2, bindx, rpntr, cpntr, bpntrb, bpntre, ublocks,
     return val, indx, route
 
 
+def sparse_to_vbr_device(A, rpntr, cpntr):
+    """convert_sparse_to_vbr's arrays built on the GPU (sable_inspect_*):
+    returns (val (FP64), indx, bindx, bpntrb, bpntre with -1 for a block row
+    without blocks, block_nnz, device_ms).  A is a CSC matrix with summed
+    duplicates."""
+    import ctypes as C
+    from . import runtime
+    from ._lib import lib, check
+    runtime.init()
+    rp = np.ascontiguousarray(rpntr, dtype=np.int64)
+    cp = np.ascontiguousarray(cpntr, dtype=np.int64)
+    ip = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(A.indices, dtype=np.int32)
+    v = np.ascontiguousarray(A.data, dtype=np.float64)
+    h, nb, nv, ms = C.c_void_p(), C.c_int64(), C.c_int64(), C.c_float()
+    check(lib.sable_inspect_create(ip.ctypes.data, ix.ctypes.data, v.ctypes.data, A.shape[0], A.shape[1],
+                                   rp.ctypes.data, len(rp) - 1, cp.ctypes.data, len(cp) - 1, C.byref(h),
+                                   C.byref(nb), C.byref(nv), C.byref(ms)))
+    try:
+        val = np.empty(nv.value, dtype=np.float64)
+        indx = np.empty(nb.value + 1, dtype=np.int64)
+        bindx = np.empty(nb.value, dtype=np.int64)
+        bb = np.empty(len(rp) - 1, dtype=np.int64)
+        be = np.empty(len(rp) - 1, dtype=np.int64)
+        bnnz = np.empty(nb.value, dtype=np.int32)
+        check(lib.sable_inspect_fetch(h, val.ctypes.data, indx.ctypes.data, bindx.ctypes.data, bb.ctypes.data,
+                                      be.ctypes.data, bnnz.ctypes.data))
+    finally:
+        lib.sable_inspect_destroy(h)
+    return val, indx, bindx, bb, be, bnnz, ms.value
+
+
 def convert_sparse_to_vbr(mat, rpntr, cpntr, fname, dst_dir, *,
-                          allow_empty_rows=False):
+                          allow_empty_rows=False, device=False):
     """utils/convert_real_to_vbr.py:98-199 (vectorised).
 
     A block is kept when it holds at least one structurally stored entry
@@ -190,16 +222,25 @@ def convert_sparse_to_vbr(mat, rpntr, cpntr, fname, dst_dir, *,
     materialised densely in column-major order.  The reference asserts that
     every block row keeps a block (:179-184); ``allow_empty_rows=True``
     writes -1/-1 for such rows instead.  Writes ``{fname}.vbr`` when
-    ``fname``/``dst_dir`` are given.
+    ``fname``/``dst_dir`` are given.  ``device=True`` builds the arrays on
+    the GPU (the same arrays, bit for bit).
     """
     A = scipy.sparse.csc_matrix(mat)
     A.sum_duplicates()
-    coo = A.tocoo()
     rp = np.asarray(rpntr, dtype=np.int64)
     cp = np.asarray(cpntr, dtype=np.int64)
     nbr, nbc = len(rp) - 1, len(cp) - 1
     logger.info("mat shape: %s, num block rows: %d, num block cols : %d, nnz: %d",
                 A.shape, len(rp), len(cp), A.nnz)
+    if device:
+        if np.iscomplexobj(A.data):
+            raise ValueError("the device path takes real values")
+        val64, indx, bindx, bpntrb, bpntre, _, _ = sparse_to_vbr_device(A, rp, cp)
+        val = val64.astype(A.dtype)
+        if np.any(bpntrb == -1) and not allow_empty_rows:
+            raise AssertionError("convert_sparse_to_vbr: a block row holds no block")
+        return _write_vbr_file(fname, dst_dir, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre)
+    coo = A.tocoo()
     i = coo.row.astype(np.int64)
     j = coo.col.astype(np.int64)
     a = np.searchsorted(rp, i, side="right") - 1
@@ -223,6 +264,10 @@ def convert_sparse_to_vbr(mat, rpntr, cpntr, fname, dst_dir, *,
     ends = np.cumsum(counts)
     bpntrb = np.where(counts > 0, ends - counts, -1).astype(np.int64)
     bpntre = np.where(counts > 0, ends, -1).astype(np.int64)
+    return _write_vbr_file(fname, dst_dir, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre)
+
+
+def _write_vbr_file(fname, dst_dir, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre):
     if fname is not None and dst_dir is not None:
         with open(os.path.join(dst_dir, f"{fname}.vbr"), "w") as f:
             for name, arr in (("val", val), ("indx", indx), ("bindx", bindx),

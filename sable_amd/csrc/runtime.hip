@@ -31,6 +31,7 @@
 #include "sable_mfma.h"
 #include "sable_mlp.h"
 #include "sable_partition.h"
+#include "sable_inspect.h"
 
 // ---------------------------------------------------------------------------
 // ahead-of-time generic instances (fallback when hipRTC is unavailable)
@@ -1326,6 +1327,179 @@ int sable_axis_cuts(const int64_t *indptr, const int32_t *indices, const double 
     if (kernel_ms) *kernel_ms = ms;
     HIPOK(hipMemcpy(cut, d_cut, nlines + 1, hipMemcpyDeviceToHost));
     if (sim && nlines > 1) HIPOK(hipMemcpy(sim, d_sim, (nlines - 1) * sizeof(double), hipMemcpyDeviceToHost));
+    return SABLE_OK;
+}
+
+// ---- block materialisation (inspector)
+
+struct SableInspect {
+    int64_t nblocks = 0, nval = 0;
+    std::vector<int64_t> bpntrb, bpntre;
+    double *d_val = nullptr;
+    int64_t *d_indx = nullptr, *d_bindx = nullptr;
+    int32_t *d_bnnz = nullptr;
+    ~SableInspect()
+    {
+        hipFree(d_val);
+        hipFree(d_indx);
+        hipFree(d_bindx);
+        hipFree(d_bnnz);
+    }
+};
+
+namespace {
+
+int check_partition(const int64_t *part, int32_t nparts, int64_t n, const char *name)
+{
+    if (!part || nparts < 1) return fail(SABLE_ERR_INVALID, std::string(name) + ": need at least one part");
+    if (part[0] != 0 || part[nparts] != n)
+        return fail(SABLE_ERR_INVALID, std::string(name) + " must start at 0 and end at the matrix size");
+    for (int32_t p = 0; p < nparts; ++p)
+        if (part[p + 1] <= part[p]) return fail(SABLE_ERR_INVALID, std::string(name) + " must strictly increase");
+    return SABLE_OK;
+}
+
+int scan_exclusive(int64_t *d_in, int64_t n, int64_t *d_out, DevBufs &b)
+{
+    const int64_t nblk = (n + SB_SCAN_BLOCK - 1) / SB_SCAN_BLOCK;
+    int64_t *d_tot = nullptr;
+    CHECKRC(b.alloc((void **)&d_tot, nblk * sizeof(int64_t)));
+    hipLaunchKernelGGL(sb_scan_blocks, dim3((unsigned)nblk), dim3(SB_SCAN_BLOCK), 0, nullptr, d_in, n, d_out, d_tot);
+    hipLaunchKernelGGL(sb_scan_totals, dim3(1), dim3(SB_SCAN_BLOCK), 0, nullptr, d_tot, nblk);
+    hipLaunchKernelGGL(sb_scan_add, dim3((unsigned)nblk), dim3(SB_SCAN_BLOCK), 0, nullptr, d_out, n, d_tot);
+    HIPOK(hipGetLastError());
+    return SABLE_OK;
+}
+
+}  // namespace
+
+int sable_inspect_create(const int64_t *indptr, const int32_t *indices, const double *data, int64_t nrows,
+                         int64_t ncols, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
+                         SableInspect **out, int64_t *nblocks, int64_t *nval, float *device_ms)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (!out || !indptr || nrows < 1 || ncols < 1) return fail(SABLE_ERR_INVALID, "bad inspect arguments");
+    *out = nullptr;
+    CHECKRC(check_partition(rpntr, nbr, nrows, "rpntr"));
+    CHECKRC(check_partition(cpntr, nbc, ncols, "cpntr"));
+    // every kernel index is bounded by these checks
+    if (indptr[0] != 0) return fail(SABLE_ERR_INVALID, "indptr[0] must be 0");
+    for (int64_t j = 0; j < ncols; ++j)
+        if (indptr[j + 1] < indptr[j]) return fail(SABLE_ERR_INVALID, "indptr must be nondecreasing");
+    const int64_t nnz = indptr[ncols];
+    if (nnz > 0 && (!indices || !data)) return fail(SABLE_ERR_INVALID, "null indices/data");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (indices[k] < 0 || indices[k] >= nrows) return fail(SABLE_ERR_INVALID, "row index out of range");
+    std::unique_ptr<SableInspect> h(new SableInspect());
+    DevBufs b;
+    const int64_t rw = (nbc + 31) / 32;
+    int64_t *d_ip = nullptr, *d_rp = nullptr, *d_cp = nullptr, *d_rowbeg = nullptr, *d_size = nullptr;
+    int32_t *d_ix = nullptr, *d_col = nullptr, *d_rmap = nullptr, *d_cmap = nullptr, *d_wpre = nullptr,
+            *d_rowcnt = nullptr;
+    double *d_v = nullptr;
+    uint32_t *d_bits = nullptr;
+    CHECKRC(b.alloc((void **)&d_ip, (ncols + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_ix, nnz * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_v, nnz * sizeof(double)));
+    CHECKRC(b.alloc((void **)&d_col, nnz * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_rp, (nbr + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_cp, (nbc + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_rmap, nrows * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_cmap, ncols * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_bits, (int64_t)nbr * rw * sizeof(uint32_t)));
+    CHECKRC(b.alloc((void **)&d_wpre, (int64_t)nbr * rw * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_rowcnt, nbr * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_rowbeg, nbr * sizeof(int64_t)));
+    HIPOK(hipMemcpy(d_ip, indptr, (ncols + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        HIPOK(hipMemcpy(d_ix, indices, nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPOK(hipMemcpy(d_v, data, nnz * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPOK(hipMemcpy(d_rp, rpntr, (nbr + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_cp, cpntr, (nbc + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    HIPOK(hipEventRecord(e0, nullptr));
+    auto g256 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    hipLaunchKernelGGL(sb_part_map, g256(nrows), dim3(256), 0, nullptr, d_rp, nbr, nrows, d_rmap);
+    hipLaunchKernelGGL(sb_part_map, g256(ncols), dim3(256), 0, nullptr, d_cp, nbc, ncols, d_cmap);
+    hipLaunchKernelGGL(sb_entry_cols, dim3((unsigned)((ncols + 3) / 4)), dim3(256), 0, nullptr, d_ip, ncols, d_col);
+    HIPOK(hipMemsetAsync(d_bits, 0, (int64_t)nbr * rw * sizeof(uint32_t), nullptr));
+    if (nnz > 0)
+        hipLaunchKernelGGL(sb_mark_blocks, g256(nnz), dim3(256), 0, nullptr, d_ix, d_col, nnz, d_rmap, d_cmap, rw,
+                           d_bits);
+    hipLaunchKernelGGL(sb_row_prefix, g256(nbr), dim3(256), 0, nullptr, d_bits, nbr, rw, d_wpre, d_rowcnt);
+    HIPOK(hipGetLastError());
+    std::vector<int32_t> rowcnt(nbr);
+    HIPOK(hipMemcpy(rowcnt.data(), d_rowcnt, nbr * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int64_t> rowbeg(nbr);
+    h->bpntrb.resize(nbr);
+    h->bpntre.resize(nbr);
+    int64_t nb = 0;
+    for (int32_t a = 0; a < nbr; ++a) {
+        rowbeg[a] = nb;
+        h->bpntrb[a] = rowcnt[a] ? nb : -1;
+        nb += rowcnt[a];
+        h->bpntre[a] = rowcnt[a] ? nb : -1;
+    }
+    h->nblocks = nb;
+    HIPOK(hipMemcpy(d_rowbeg, rowbeg.data(), nbr * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMalloc((void **)&h->d_bindx, std::max<int64_t>(nb, 1) * sizeof(int64_t)));
+    HIPOK(hipMalloc((void **)&h->d_indx, (nb + 1) * sizeof(int64_t)));
+    HIPOK(hipMalloc((void **)&h->d_bnnz, std::max<int64_t>(nb, 1) * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_size, nb * sizeof(int64_t)));
+    int64_t nv = 0;
+    if (nb > 0) {
+        hipLaunchKernelGGL(sb_kept_blocks, g256((int64_t)nbr * rw), dim3(256), 0, nullptr, d_bits, d_wpre, d_rowbeg,
+                           nbr, rw, d_rp, d_cp, h->d_bindx, d_size);
+        CHECKRC(scan_exclusive(d_size, nb, h->d_indx, b));
+        int64_t last[2] = {0, 0};
+        HIPOK(hipMemcpy(&last[0], h->d_indx + nb - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPOK(hipMemcpy(&last[1], d_size + nb - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+        nv = last[0] + last[1];
+    }
+    HIPOK(hipMemcpy(h->d_indx + nb, &nv, sizeof(int64_t), hipMemcpyHostToDevice));
+    h->nval = nv;
+    HIPOK(hipMalloc((void **)&h->d_val, std::max<int64_t>(nv, 1) * sizeof(double)));
+    HIPOK(hipMemsetAsync(h->d_val, 0, std::max<int64_t>(nv, 1) * sizeof(double), nullptr));
+    HIPOK(hipMemsetAsync(h->d_bnnz, 0, std::max<int64_t>(nb, 1) * sizeof(int32_t), nullptr));
+    if (nnz > 0)
+        hipLaunchKernelGGL(sb_scatter_values, g256(nnz), dim3(256), 0, nullptr, d_ix, d_col, d_v, nnz, d_rmap,
+                           d_cmap, d_bits, d_wpre, d_rowbeg, rw, d_rp, d_cp, h->d_indx, h->d_val, h->d_bnnz);
+    HIPOK(hipEventRecord(e1, nullptr));
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPOK(le);
+    HIPOK(se);
+    if (device_ms) *device_ms = ms;
+    if (nblocks) *nblocks = h->nblocks;
+    if (nval) *nval = h->nval;
+    *out = h.release();
+    return SABLE_OK;
+}
+
+int sable_inspect_fetch(SableInspect *h, double *val, int64_t *indx, int64_t *bindx, int64_t *bpntrb,
+                        int64_t *bpntre, int32_t *block_nnz)
+{
+    if (!h) return fail(SABLE_ERR_INVALID, "null inspect handle");
+    if (val && h->nval) HIPOK(hipMemcpy(val, h->d_val, h->nval * sizeof(double), hipMemcpyDeviceToHost));
+    if (indx) HIPOK(hipMemcpy(indx, h->d_indx, (h->nblocks + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (bindx && h->nblocks) HIPOK(hipMemcpy(bindx, h->d_bindx, h->nblocks * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (block_nnz && h->nblocks)
+        HIPOK(hipMemcpy(block_nnz, h->d_bnnz, h->nblocks * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (bpntrb) std::copy(h->bpntrb.begin(), h->bpntrb.end(), bpntrb);
+    if (bpntre) std::copy(h->bpntre.begin(), h->bpntre.end(), bpntre);
+    return SABLE_OK;
+}
+
+int sable_inspect_destroy(SableInspect *h)
+{
+    delete h;
     return SABLE_OK;
 }
 

@@ -1,0 +1,185 @@
+// Block materialisation on the device (reference:
+// utils/convert_real_to_vbr.py:98-199, convert_sparse_to_vbr): from a CSC
+// matrix and the row / column partitions to the VBR arrays.
+//
+// A block (a, b) is kept when it holds at least one stored entry; kept
+// blocks are numbered in block-row-major order and stored densely in
+// column-major order.  On the device:
+//   1. row -> block row and column -> block column maps (binary searches of
+//      rpntr / cpntr), and the column of every entry;
+//   2. a bit grid of the kept blocks: row a of the grid is rw = ceil(nbc/32)
+//      32-bit words, set with atomicOr per entry;
+//   3. per block row, the count of kept blocks before each word (exclusive,
+//      within the row) and the row's total; bpntrb / bpntre follow on the
+//      host from the row totals (nbr values);
+//   4. the ordinal of a kept block = bpntrb[a] + word prefix + popcount of
+//      the lower bits of its word; per kept block its bindx and its size
+//      h * w, whose exclusive scan is indx;
+//   5. every entry is written to val[indx[ord] + (j - c0) h + (i - r0)]
+//      (val zeroed first), and the nonzero values are counted per block.
+// Integer work and value copies only: the result equals the host
+// restatement bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// line -> partition index: the last p with part[p] <= line
+__global__ void __launch_bounds__(256) sb_part_map(const int64_t *__restrict__ part, int32_t nparts, int64_t n,
+                                                   int32_t *__restrict__ map)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t lo = 0, hi = nparts;                      // part[lo] <= i < part[hi]
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (part[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    map[i] = lo;
+}
+
+// column of every entry (one wavefront per column)
+__global__ void __launch_bounds__(256) sb_entry_cols(const int64_t *__restrict__ ip, int64_t ncols,
+                                                     int32_t *__restrict__ col)
+{
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= ncols) return;
+    for (int64_t k = ip[j] + (threadIdx.x & 63); k < ip[j + 1]; k += 64) col[k] = (int32_t)j;
+}
+
+__global__ void __launch_bounds__(256) sb_mark_blocks(const int32_t *__restrict__ ix,
+                                                      const int32_t *__restrict__ col, int64_t nnz,
+                                                      const int32_t *__restrict__ rmap,
+                                                      const int32_t *__restrict__ cmap, int64_t rw,
+                                                      uint32_t *__restrict__ bits)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int64_t a = rmap[ix[k]], b = cmap[col[k]];
+    atomicOr(&bits[a * rw + (b >> 5)], 1u << (b & 31));
+}
+
+// per block row: exclusive count of kept blocks before each word of the row,
+// and the row total
+__global__ void __launch_bounds__(256) sb_row_prefix(const uint32_t *__restrict__ bits, int32_t nbr, int64_t rw,
+                                                     int32_t *__restrict__ wpre, int32_t *__restrict__ rowcnt)
+{
+    const int32_t a = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (a >= nbr) return;
+    int32_t run = 0;
+    for (int64_t w = 0; w < rw; ++w) {
+        wpre[(int64_t)a * rw + w] = run;
+        run += __popc(bits[(int64_t)a * rw + w]);
+    }
+    rowcnt[a] = run;
+}
+
+__device__ __forceinline__ int64_t sb_block_ordinal(const uint32_t *bits, const int32_t *wpre,
+                                                    const int64_t *rowbeg, int64_t rw, int64_t a, int64_t b)
+{
+    const int64_t w = a * rw + (b >> 5);
+    const uint32_t below = (b & 31) ? (bits[w] & ((1u << (b & 31)) - 1u)) : 0u;
+    return rowbeg[a] + wpre[w] + __popc(below);
+}
+
+// one thread per grid word: bindx and size of every kept block in it
+__global__ void __launch_bounds__(256) sb_kept_blocks(const uint32_t *__restrict__ bits,
+                                                      const int32_t *__restrict__ wpre,
+                                                      const int64_t *__restrict__ rowbeg, int32_t nbr, int64_t rw,
+                                                      const int64_t *__restrict__ rpntr,
+                                                      const int64_t *__restrict__ cpntr,
+                                                      int64_t *__restrict__ bindx, int64_t *__restrict__ size)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (int64_t)nbr * rw) return;
+    const int64_t a = w / rw;
+    uint32_t m = bits[w];
+    int64_t ord = rowbeg[a] + wpre[w];
+    const int64_t h = rpntr[a + 1] - rpntr[a];
+    while (m) {
+        const int bit = __ffs(m) - 1;
+        m &= m - 1u;
+        const int64_t b = (w - a * rw) * 32 + bit;
+        bindx[ord] = b;
+        size[ord] = h * (cpntr[b + 1] - cpntr[b]);
+        ++ord;
+    }
+}
+
+// exclusive scan of int64 values in three launches: per-block scans with
+// block totals, a scan of the totals in one block, then the block offsets
+#define SB_SCAN_BLOCK 1024
+
+__global__ void __launch_bounds__(SB_SCAN_BLOCK) sb_scan_blocks(const int64_t *__restrict__ in, int64_t n,
+                                                                int64_t *__restrict__ out, int64_t *__restrict__ tot)
+{
+    __shared__ int64_t s[SB_SCAN_BLOCK];
+    const int64_t g = (int64_t)blockIdx.x * SB_SCAN_BLOCK + threadIdx.x;
+    const int64_t x = g < n ? in[g] : 0;
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 1; o < SB_SCAN_BLOCK; o <<= 1) {     // Hillis-Steele, inclusive
+        const int64_t y = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += y;
+        __syncthreads();
+    }
+    if (g < n) out[g] = s[threadIdx.x] - x;
+    if (threadIdx.x == SB_SCAN_BLOCK - 1) tot[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(SB_SCAN_BLOCK) sb_scan_totals(int64_t *__restrict__ tot, int64_t nb)
+{
+    // one block: serial carry over chunks of SB_SCAN_BLOCK totals
+    __shared__ int64_t s[SB_SCAN_BLOCK];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < nb; c0 += SB_SCAN_BLOCK) {
+        const int64_t g = c0 + threadIdx.x;
+        const int64_t x = g < nb ? tot[g] : 0;
+        s[threadIdx.x] = x;
+        __syncthreads();
+        for (int o = 1; o < SB_SCAN_BLOCK; o <<= 1) {
+            const int64_t y = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+            __syncthreads();
+            s[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (g < nb) tot[g] = carry + s[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == SB_SCAN_BLOCK - 1) carry += s[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(SB_SCAN_BLOCK) sb_scan_add(int64_t *__restrict__ out, int64_t n,
+                                                             const int64_t *__restrict__ tot)
+{
+    const int64_t g = (int64_t)blockIdx.x * SB_SCAN_BLOCK + threadIdx.x;
+    if (g < n) out[g] += tot[blockIdx.x];
+}
+
+__global__ void __launch_bounds__(256) sb_scatter_values(const int32_t *__restrict__ ix,
+                                                         const int32_t *__restrict__ col,
+                                                         const double *__restrict__ v, int64_t nnz,
+                                                         const int32_t *__restrict__ rmap,
+                                                         const int32_t *__restrict__ cmap,
+                                                         const uint32_t *__restrict__ bits,
+                                                         const int32_t *__restrict__ wpre,
+                                                         const int64_t *__restrict__ rowbeg, int64_t rw,
+                                                         const int64_t *__restrict__ rpntr,
+                                                         const int64_t *__restrict__ cpntr,
+                                                         const int64_t *__restrict__ indx,
+                                                         double *__restrict__ val, int32_t *__restrict__ bnnz)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int64_t i = ix[k], j = col[k];
+    const int64_t a = rmap[i], b = cmap[j];
+    const int64_t ord = sb_block_ordinal(bits, wpre, rowbeg, rw, a, b);
+    const int64_t h = rpntr[a + 1] - rpntr[a];
+    val[indx[ord] + (j - cpntr[b]) * h + (i - rpntr[a])] = v[k];
+    if (v[k] != 0.0) atomicAdd(&bnnz[ord], 1);
+}
