@@ -17,7 +17,7 @@ per launch / measured kernel time vs 8 TB/s HBM), both plans' figures, and
 the reference CPU path (the restated generated-C program, FP64 like the
 reference) timed on this host's cores on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5|partition] [--warm]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5|partition|inspect] [--warm]
                     [--routing plain|learned|both] [--no-cpu]
 """
 from __future__ import annotations
@@ -348,6 +348,63 @@ def bench_partition(args, runtime, info):
     print(json.dumps(out), flush=True)
 
 
+def bench_inspect(args, runtime, info):
+    """Inspector workload (utils/convert_real_to_vbr.py:98-199,
+    convert_sparse_to_vbr): the 100,000 x 100,000 SpMV matrix as CSC (its
+    51 M stored nonzeros, FP64) and its 1000 x 1000 partitions to the VBR
+    arrays on the GPU.  Time: the device pipeline after the input upload
+    (sable_inspect_create); the arrays are checked against the host
+    restatement bit for bit.  CPU baseline: the oracle's per-entry
+    restatement on the first 30 x 30 blocks, as entries per second."""
+    from sable_amd import synth
+    from sable_amd import convert as CV
+    vbr, _, _ = synth.workload_c2()
+    A = CV.vbr_to_csc(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    rp, cp = np.asarray(vbr.rpntr, np.int64), np.asarray(vbr.cpntr, np.int64)
+    del vbr
+    log(f"inspect: {A.shape[0]}x{A.shape[1]} matrix, {A.nnz} nonzeros, {len(rp) - 1}x{len(cp) - 1} partitions")
+    times = []
+    for rep in range(args.warmup + args.steps):
+        out = CV.sparse_to_vbr_device(A, rp, cp)
+        if rep >= args.warmup:
+            times.append(out[-1])
+    val, indx, bindx, bb, be, _, _ = out
+    t0 = time.time()
+    host = CV.convert_sparse_to_vbr(A, rp, cp, None, None, allow_empty_rows=True)
+    host_s = time.time() - t0
+    assert all(np.array_equal(a, b) for a, b in zip(host, (val, indx, bindx, bb, be))), "device != host"
+    ms = float(np.mean(times))
+    bytes_alg = 12 * A.nnz + 8 * (A.shape[1] + 1) + 8 * len(val) + 16 * len(bindx) + 16 * (len(rp) - 1)
+    res = {
+        "metric": "inspector (convert_sparse_to_vbr): CSC + partitions to VBR arrays",
+        "value": bytes_alg / (ms * 1e-3) / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (values), int64/int32 (indices)", "data": "synthetic",
+        "config": {"workload": f"convert_sparse_to_vbr of the 100000x100000 SpMV matrix ({A.nnz} nonzeros, "
+                               f"{len(rp) - 1}x{len(cp) - 1} partitions, {len(bindx)} blocks kept)"},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": bytes_alg / (ms * 1e-3) / 1e9 / 8000.0, "traffic": None},
+        "bytes_alg": bytes_alg, "entries_per_s": A.nnz / (ms * 1e-3), "host_restatement_s": host_s,
+        "matches_host": True, "cpu_baseline": None, "device": info,
+    }
+    if not args.no_cpu:
+        from oracle import vbr_oracle
+        kr = kc = 30
+        S = A[:int(rp[kr]), :int(cp[kc])].tocoo()
+        entries = {(int(i), int(j)): float(v) for i, j, v in zip(S.row, S.col, S.data)}
+        t0 = time.time()
+        vbr_oracle.convert_sparse_to_vbr(entries, S.shape, rp[:kr + 1].tolist(), cp[:kc + 1].tolist(),
+                                         allow_empty_rows=True)
+        dt = time.time() - t0
+        res["cpu_baseline"] = {"value": len(entries) / dt, "unit": "entries/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's convert_sparse_to_vbr (per-entry restatement) on the first "
+                                         f"{kr}x{kc} blocks ({len(entries)} entries, {dt:.1f} s); its cost "
+                                         f"grows faster than the entry count, so the full-matrix figure is "
+                                         f"a lower bound",
+                               "est_full_matrix_s_lower_bound": A.nnz / (len(entries) / dt)}
+    print(json.dumps(res), flush=True)
+
+
 def spmv_options(args):
     """Plan options: the autotuned table's winners unless given on the command line."""
     from sable_amd.autotune import tuned_options
@@ -363,7 +420,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "partition"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "partition", "inspect"])
     ap.add_argument("--count", type=int, default=256, help="c5: number of matrices")
     ap.add_argument("--warm", action="store_true", help="no cache flush between launches")
     ap.add_argument("--routing", default="both", choices=["plain", "learned", "both"],
@@ -399,6 +456,8 @@ def main():
         return bench_batch(args, runtime, info)
     elif args.workload == "partition":
         return bench_partition(args, runtime, info)
+    elif args.workload == "inspect":
+        return bench_inspect(args, runtime, info)
     else:
         return bench_spmm(args, runtime, info)
     log(f"workload generated in {time.time() - t0:.1f}s: {len(vbr.val)} stored values, {len(vbr.bindx)} blocks")

@@ -8,7 +8,8 @@
 //   1. row -> block row and column -> block column maps (binary searches of
 //      rpntr / cpntr), and the column of every entry;
 //   2. a bit grid of the kept blocks: row a of the grid is rw = ceil(nbc/32)
-//      32-bit words, set with atomicOr per entry;
+//      32-bit words, set with atomicOr by the first entry of every run of a
+//      column's entries in one block row;
 //   3. per block row, the count of kept blocks before each word (exclusive,
 //      within the row) and the row's total; bpntrb / bpntre follow on the
 //      host from the row totals (nbr values);
@@ -16,7 +17,8 @@
 //      the lower bits of its word; per kept block its bindx and its size
 //      h * w, whose exclusive scan is indx;
 //   5. every entry is written to val[indx[ord] + (j - c0) h + (i - r0)]
-//      (val zeroed first), and the nonzero values are counted per block.
+//      (val zeroed first), and the nonzero values are counted per block
+//      (one atomic add per run).
 // Integer work and value copies only: the result equals the host
 // restatement bit for bit.
 #pragma once
@@ -56,7 +58,12 @@ __global__ void __launch_bounds__(256) sb_mark_blocks(const int32_t *__restrict_
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nnz) return;
-    const int64_t a = rmap[ix[k]], b = cmap[col[k]];
+    const int32_t j = col[k];
+    const int64_t a = rmap[ix[k]];
+    // a column's entries are in row order: only the first entry of each run
+    // in one block row marks the block (one atomic per run, not per entry)
+    if (k > 0 && col[k - 1] == j && rmap[ix[k - 1]] == a) return;
+    const int64_t b = cmap[j];
     atomicOr(&bits[a * rw + (b >> 5)], 1u << (b & 31));
 }
 
@@ -181,5 +188,10 @@ __global__ void __launch_bounds__(256) sb_scatter_values(const int32_t *__restri
     const int64_t ord = sb_block_ordinal(bits, wpre, rowbeg, rw, a, b);
     const int64_t h = rpntr[a + 1] - rpntr[a];
     val[indx[ord] + (j - cpntr[b]) * h + (i - rpntr[a])] = v[k];
-    if (v[k] != 0.0) atomicAdd(&bnnz[ord], 1);
+    // the first entry of a run (same column, same block row) counts the
+    // run's nonzero values and adds them once
+    if (k > 0 && col[k - 1] == j && rmap[ix[k - 1]] == a) return;
+    int32_t nzr = 0;
+    for (int64_t q = k; q < nnz && col[q] == j && rmap[ix[q]] == a; ++q) nzr += v[q] != 0.0;
+    if (nzr) atomicAdd(&bnnz[ord], nzr);
 }
