@@ -54,6 +54,7 @@ extern "C" {
 typedef struct SablePlan SablePlan;
 typedef struct SableGroup SableGroup;
 typedef struct SableInspect SableInspect;
+typedef struct SableCompress SableCompress;
 
 /* The VBR arrays of src/vbr.py:7-14 (validated: see sable_plan_create). */
 typedef struct {
@@ -245,6 +246,27 @@ int sable_inspect_create(const int64_t *indptr, const int32_t *indices, const do
 int sable_inspect_fetch(SableInspect *h, double *val, int64_t *indx, int64_t *bindx, int64_t *bpntrb,
                         int64_t *bpntre, int32_t *block_nnz);
 int sable_inspect_destroy(SableInspect *h);
+
+/* Compressed (.vbrc) arrays on the device for given per-block routes
+ * (utils/convert_real_to_vbr.py:18-96, convert_vbr_to_compressed, after its
+ * routing step): dense-routed blocks are gathered into val2 / indx2, the
+ * ordinals of sparse-routed blocks are ublocks, and their nonzero values form
+ * a CSR matrix with sorted columns and the reference's padding entry (row
+ * nrows - 1, column ncols - 1, value 0, when the last row or column holds no
+ * entry).  Inputs are the VBR arrays in host memory (FP64 val) and one
+ * route byte per block (SABLE_ROUTE_*).  create returns the sizes
+ * (csr_nnz = -1 when there is no CSR part: no sparse-routed block or no
+ * nonzero in them, as the reference writes none) and the device time after
+ * the upload; fetch copies the arrays out (indptr nrows + 1, indices and
+ * csr_val csr_nnz entries); any output may be null. */
+int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, const int64_t *bindx,
+                          int64_t nblocks, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
+                          const int64_t *bpntrb, const int64_t *bpntre, const uint8_t *routes,
+                          SableCompress **out, int64_t *nval2, int64_t *nublocks, int64_t *csr_nnz,
+                          float *device_ms);
+int sable_compress_fetch(SableCompress *h, double *val2, int64_t *indx2, int64_t *ublocks, int64_t *indptr,
+                         int64_t *indices, double *csr_val);
+int sable_compress_destroy(SableCompress *h);
 
 /* Device memory helpers for callers without another allocator. */
 int sable_malloc(void **ptr, int64_t bytes);

@@ -114,6 +114,10 @@ _SIGS = {
                                    C.POINTER(I64), F32P]),
     "sable_inspect_fetch": (I32, [P, P, P, P, P, P, P]),
     "sable_inspect_destroy": (I32, [P]),
+    "sable_compress_create": (I32, [P, I64, P, P, I64, P, I32, P, I32, P, P, P, C.POINTER(P), C.POINTER(I64),
+                                    C.POINTER(I64), C.POINTER(I64), F32P]),
+    "sable_compress_fetch": (I32, [P, P, P, P, P, P, P]),
+    "sable_compress_destroy": (I32, [P]),
     "sable_malloc": (I32, [C.POINTER(P), I64]),
     "sable_free": (I32, [P]),
     "sable_memcpy_h2d": (I32, [P, P, I64]),

@@ -109,9 +109,62 @@ def compress_with_routes(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, routes)
     return val2, indx2, ublocks, csr.indptr, csr.indices, csr.data
 
 
+def compress_with_routes_device(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, routes, return_ms=False):
+    """compress_with_routes on the GPU (sable_compress_*): the same arrays,
+    dtypes included, bit for bit."""
+    import ctypes as C
+    from . import runtime
+    from ._lib import lib, check
+    runtime.init()
+    val = np.asarray(val)
+    v64 = np.ascontiguousarray(val, dtype=np.float64)
+    ix = np.ascontiguousarray(indx, dtype=np.int64)
+    bx = np.ascontiguousarray(bindx, dtype=np.int64)
+    rp = np.ascontiguousarray(rpntr, dtype=np.int64)
+    cp = np.ascontiguousarray(cpntr, dtype=np.int64)
+    bb = np.ascontiguousarray(bpntrb, dtype=np.int64)
+    be = np.ascontiguousarray(bpntre, dtype=np.int64)
+    rt = np.ascontiguousarray(routes, dtype=np.uint8)
+    nb = len(ix) - 1
+    h, nv2, nub, nnz, ms = C.c_void_p(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_float()
+    check(lib.sable_compress_create(v64.ctypes.data, len(v64), ix.ctypes.data, bx.ctypes.data, nb, rp.ctypes.data,
+                                    len(rp) - 1, cp.ctypes.data, len(cp) - 1, bb.ctypes.data, be.ctypes.data,
+                                    rt.ctypes.data, C.byref(h), C.byref(nv2), C.byref(nub), C.byref(nnz),
+                                    C.byref(ms)))
+    try:
+        val2 = np.empty(nv2.value, dtype=np.float64)
+        indx2 = np.empty(nb + 1 - nub.value, dtype=np.int64)
+        ub = np.empty(nub.value, dtype=np.int64)
+        nrows = int(rp[-1])
+        if nnz.value >= 0:
+            ip = np.empty(nrows + 1, dtype=np.int64)
+            ind = np.empty(nnz.value, dtype=np.int64)
+            cv = np.empty(nnz.value, dtype=np.float64)
+            check(lib.sable_compress_fetch(h, val2.ctypes.data, indx2.ctypes.data, ub.ctypes.data, ip.ctypes.data,
+                                           ind.ctypes.data, cv.ctypes.data))
+        else:
+            check(lib.sable_compress_fetch(h, val2.ctypes.data, indx2.ctypes.data, ub.ctypes.data, None, None,
+                                           None))
+    finally:
+        lib.sable_compress_destroy(h)
+    val2 = val2.astype(val.dtype)
+    if nnz.value < 0:
+        out = (val2, indx2, ub, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.float64))
+    else:
+        # the index dtypes scipy gives the host construction (COO of int64
+        # row / column arrays -> CSR), and the float64 the padding entry
+        # brings in
+        probe = scipy.sparse.coo_array((np.zeros(1), (np.array([nrows - 1], dtype=np.int64),
+                                                      np.array([int(cp[-1]) - 1], dtype=np.int64)))).tocsr()
+        padded = cv[-1] == 0.0
+        out = (val2, indx2, ub, ip.astype(probe.indptr.dtype), ind.astype(probe.indices.dtype),
+               cv if padded else cv.astype(val.dtype))
+    return (out, ms.value) if return_ms else out
+
+
 def convert_vbr_to_compressed(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre,
                               fname, dst_dir, density=None, *, router=None,
-                              routes=None):
+                              routes=None, device=False):
     """utils/convert_real_to_vbr.py:18-96 (vectorised).
 
     ``density``: fixed threshold T (percent); a block is dense iff its
@@ -119,7 +172,8 @@ def convert_vbr_to_compressed(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre,
     learned ``router`` decides (falling back to ``DEFAULT_DENSITY`` when no
     router is given).  ``routes`` overrides both with explicit per-block
     routes.  The .vbrc file is written when ``fname`` and ``dst_dir`` are not
-    None, as in the reference.
+    None, as in the reference.  ``device=True`` assembles the arrays on the
+    GPU (the same arrays, bit for bit).
     """
     validate_vbr(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre)
     if routes is None:
@@ -130,7 +184,8 @@ def convert_vbr_to_compressed(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre,
                                          bpntrb, bpntre, n_rhs=1)
         else:
             routes = threshold_routes(val, indx, DEFAULT_DENSITY)
-    val2, indx2, ublocks, indptr, indices, csr_val = compress_with_routes(
+    compress = compress_with_routes_device if device else compress_with_routes
+    val2, indx2, ublocks, indptr, indices, csr_val = compress(
         val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, routes)
     if fname is not None and dst_dir is not None:
         os.makedirs(dst_dir, exist_ok=True)

@@ -1503,6 +1503,178 @@ int sable_inspect_destroy(SableInspect *h)
     return SABLE_OK;
 }
 
+// ---- compressed (.vbrc) arrays for given routes
+
+struct SableCompress {
+    int64_t nval2 = 0, nub = 0, nnz = 0;   // nnz: CSR entries including the padding entry
+    bool has_csr = false;
+    std::vector<int64_t> indx2, ublocks;
+    double *d_val2 = nullptr, *d_vals = nullptr;
+    int64_t *d_rowptr = nullptr, *d_cols = nullptr;
+    int64_t nrows = 0;
+    ~SableCompress()
+    {
+        hipFree(d_val2);
+        hipFree(d_vals);
+        hipFree(d_rowptr);
+        hipFree(d_cols);
+    }
+};
+
+int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, const int64_t *bindx,
+                          int64_t nblocks, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
+                          const int64_t *bpntrb, const int64_t *bpntre, const uint8_t *routes,
+                          SableCompress **out, int64_t *nval2, int64_t *nublocks, int64_t *csr_nnz,
+                          float *device_ms)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (!out || !indx || !bpntrb || !bpntre || nblocks < 0 || nval < 0) return fail(SABLE_ERR_INVALID, "bad compress arguments");
+    *out = nullptr;
+    if (nblocks > 0 && (!val || !bindx || !routes)) return fail(SABLE_ERR_INVALID, "null val/bindx/routes");
+    const int64_t nrows = rpntr ? rpntr[nbr] : 0, ncols = cpntr ? cpntr[nbc] : 0;
+    CHECKRC(check_partition(rpntr, nbr, nrows, "rpntr"));
+    CHECKRC(check_partition(cpntr, nbc, ncols, "cpntr"));
+    // the kernels read block k only inside [indx[k], indx[k] + h w): check it
+    if (indx[0] != 0 || indx[nblocks] != nval) return fail(SABLE_ERR_INVALID, "indx must run from 0 to len(val)");
+    int64_t next = 0;
+    for (int32_t a = 0; a < nbr; ++a) {
+        if (bpntrb[a] == -1 && bpntre[a] == -1) continue;
+        if (bpntrb[a] != next || bpntre[a] < bpntrb[a] || bpntre[a] > nblocks)
+            return fail(SABLE_ERR_INVALID, "bpntrb/bpntre must lay the block rows out in order");
+        const int64_t h = rpntr[a + 1] - rpntr[a];
+        for (int64_t k = bpntrb[a]; k < bpntre[a]; ++k) {
+            if (bindx[k] < 0 || bindx[k] >= nbc) return fail(SABLE_ERR_INVALID, "bindx out of range");
+            if (indx[k + 1] - indx[k] != h * (cpntr[bindx[k] + 1] - cpntr[bindx[k]]))
+                return fail(SABLE_ERR_INVALID, "indx step must equal the block size");
+        }
+        next = bpntre[a];
+    }
+    if (next != nblocks) return fail(SABLE_ERR_INVALID, "bpntre must end at the block count");
+    std::unique_ptr<SableCompress> h(new SableCompress());
+    h->nrows = nrows;
+    std::vector<int64_t> dblk;
+    std::vector<uint8_t> sp(std::max<int64_t>(nblocks, 1), 0);
+    h->indx2.push_back(0);
+    for (int64_t k = 0; k < nblocks; ++k) {
+        if (routes[k] == SABLE_ROUTE_SPARSE) {
+            sp[k] = 1;
+            h->ublocks.push_back(k);
+        } else {
+            dblk.push_back(k);
+            h->indx2.push_back(h->indx2.back() + indx[k + 1] - indx[k]);
+        }
+    }
+    h->nval2 = h->indx2.back();
+    h->nub = (int64_t)h->ublocks.size();
+    DevBufs b;
+    double *d_val = nullptr;
+    int64_t *d_indx = nullptr, *d_bindx = nullptr, *d_rp = nullptr, *d_cp = nullptr, *d_bb = nullptr,
+            *d_be = nullptr, *d_dblk = nullptr, *d_indx2 = nullptr, *d_cnt = nullptr;
+    uint8_t *d_sp = nullptr;
+    int32_t *d_rmap = nullptr, *d_last = nullptr;
+    CHECKRC(b.alloc((void **)&d_val, nval * sizeof(double)));
+    CHECKRC(b.alloc((void **)&d_indx, (nblocks + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_bindx, nblocks * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_rp, (nbr + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_cp, (nbc + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_bb, nbr * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_be, nbr * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_dblk, dblk.size() * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_indx2, h->indx2.size() * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_cnt, nrows * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_sp, sp.size()));
+    CHECKRC(b.alloc((void **)&d_rmap, nrows * sizeof(int32_t)));
+    CHECKRC(b.alloc((void **)&d_last, sizeof(int32_t)));
+    if (nval) HIPOK(hipMemcpy(d_val, val, nval * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_indx, indx, (nblocks + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (nblocks) HIPOK(hipMemcpy(d_bindx, bindx, nblocks * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_rp, rpntr, (nbr + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_cp, cpntr, (nbc + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_bb, bpntrb, nbr * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_be, bpntre, nbr * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (!dblk.empty()) HIPOK(hipMemcpy(d_dblk, dblk.data(), dblk.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_indx2, h->indx2.data(), h->indx2.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_sp, sp.data(), sp.size(), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    HIPOK(hipEventRecord(e0, nullptr));
+    auto g256 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    HIPOK(hipMalloc((void **)&h->d_val2, std::max<int64_t>(h->nval2, 1) * sizeof(double)));
+    if (!dblk.empty())
+        hipLaunchKernelGGL(sb_gather_dense, dim3((unsigned)((dblk.size() + 3) / 4)), dim3(256), 0, nullptr, d_val,
+                           d_indx, d_dblk, d_indx2, (int64_t)dblk.size(), h->d_val2);
+    int64_t nnz = 0;
+    if (h->nub > 0) {
+        hipLaunchKernelGGL(sb_part_map, g256(nrows), dim3(256), 0, nullptr, d_rp, nbr, nrows, d_rmap);
+        HIPOK(hipMemsetAsync(d_last, 0, sizeof(int32_t), nullptr));
+        hipLaunchKernelGGL(sb_csr_rows, g256(nrows), dim3(256), 0, nullptr, d_val, d_indx, d_bindx, d_rp, d_cp,
+                           d_bb, d_be, d_sp, d_rmap, nrows, ncols, d_cnt, nullptr, nullptr, nullptr, d_last);
+        HIPOK(hipMalloc((void **)&h->d_rowptr, (nrows + 1) * sizeof(int64_t)));
+        CHECKRC(scan_exclusive(d_cnt, nrows, h->d_rowptr, b));
+        int64_t lastptr = 0, lastcnt = 0;
+        int32_t lastcol = 0;
+        HIPOK(hipMemcpy(&lastptr, h->d_rowptr + nrows - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPOK(hipMemcpy(&lastcnt, d_cnt + nrows - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPOK(hipMemcpy(&lastcol, d_last, sizeof(int32_t), hipMemcpyDeviceToHost));
+        nnz = lastptr + lastcnt;
+        if (nnz > 0) {
+            // the reference's padding entry (utils/convert_real_to_vbr.py:75-80)
+            const bool pad = !(lastcnt > 0 && lastcol);
+            h->has_csr = true;
+            h->nnz = nnz + (pad ? 1 : 0);
+            HIPOK(hipMalloc((void **)&h->d_cols, h->nnz * sizeof(int64_t)));
+            HIPOK(hipMalloc((void **)&h->d_vals, h->nnz * sizeof(double)));
+            hipLaunchKernelGGL(sb_csr_rows, g256(nrows), dim3(256), 0, nullptr, d_val, d_indx, d_bindx, d_rp, d_cp,
+                               d_bb, d_be, d_sp, d_rmap, nrows, ncols, d_cnt, h->d_rowptr, h->d_cols, h->d_vals,
+                               d_last);
+            if (pad) {
+                const int64_t c = ncols - 1;
+                const double z = 0.0;
+                HIPOK(hipMemcpy(h->d_cols + nnz, &c, sizeof(int64_t), hipMemcpyHostToDevice));
+                HIPOK(hipMemcpy(h->d_vals + nnz, &z, sizeof(double), hipMemcpyHostToDevice));
+            }
+            HIPOK(hipMemcpy(h->d_rowptr + nrows, &h->nnz, sizeof(int64_t), hipMemcpyHostToDevice));
+        }
+    }
+    HIPOK(hipEventRecord(e1, nullptr));
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPOK(le);
+    HIPOK(se);
+    if (device_ms) *device_ms = ms;
+    if (nval2) *nval2 = h->nval2;
+    if (nublocks) *nublocks = h->nub;
+    if (csr_nnz) *csr_nnz = h->has_csr ? h->nnz : -1;
+    *out = h.release();
+    return SABLE_OK;
+}
+
+int sable_compress_fetch(SableCompress *h, double *val2, int64_t *indx2, int64_t *ublocks, int64_t *indptr,
+                         int64_t *indices, double *csr_val)
+{
+    if (!h) return fail(SABLE_ERR_INVALID, "null compress handle");
+    if (val2 && h->nval2) HIPOK(hipMemcpy(val2, h->d_val2, h->nval2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (indx2) std::copy(h->indx2.begin(), h->indx2.end(), indx2);
+    if (ublocks) std::copy(h->ublocks.begin(), h->ublocks.end(), ublocks);
+    if (h->has_csr) {
+        if (indptr) HIPOK(hipMemcpy(indptr, h->d_rowptr, (h->nrows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (indices) HIPOK(hipMemcpy(indices, h->d_cols, h->nnz * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (csr_val) HIPOK(hipMemcpy(csr_val, h->d_vals, h->nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return SABLE_OK;
+}
+
+int sable_compress_destroy(SableCompress *h)
+{
+    delete h;
+    return SABLE_OK;
+}
+
 // ---- device memory helpers
 
 int sable_malloc(void **ptr, int64_t bytes)

@@ -195,3 +195,68 @@ __global__ void __launch_bounds__(256) sb_scatter_values(const int32_t *__restri
     for (int64_t q = k; q < nnz && col[q] == j && rmap[ix[q]] == a; ++q) nzr += v[q] != 0.0;
     if (nzr) atomicAdd(&bnnz[ord], nzr);
 }
+
+// ---- compressed (.vbrc) arrays on the device (reference:
+// utils/convert_real_to_vbr.py:18-96, convert_vbr_to_compressed, for given
+// routes).  Dense-routed blocks are gathered into val2 (one wavefront per
+// block); the nonzero values of sparse-routed blocks become a CSR matrix.
+// One thread per row walks its block row's sparse blocks in bindx order and,
+// inside a block, its columns in order, so every row comes out sorted by
+// column, as scipy's COO -> CSR conversion leaves it.  The reference's
+// padding entry (row nrows - 1, column ncols - 1, value 0, when the last row
+// or the last column holds no entry) is appended by the host.
+
+__global__ void __launch_bounds__(256) sb_gather_dense(const double *__restrict__ val,
+                                                       const int64_t *__restrict__ indx,
+                                                       const int64_t *__restrict__ dblk,
+                                                       const int64_t *__restrict__ indx2, int64_t ndense,
+                                                       double *__restrict__ val2)
+{
+    const int64_t d = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (d >= ndense) return;
+    const int64_t k = dblk[d], src = indx[k], n = indx[k + 1] - src, dst = indx2[d];
+    for (int64_t e = threadIdx.x & 63; e < n; e += 64) val2[dst + e] = val[src + e];
+}
+
+// per row: the nonzero values of its sparse-routed blocks (count pass when
+// cols == nullptr, fill pass otherwise); lastcol is set when one of them is
+// in the last column
+__global__ void __launch_bounds__(256) sb_csr_rows(const double *__restrict__ val,
+                                                   const int64_t *__restrict__ indx,
+                                                   const int64_t *__restrict__ bindx,
+                                                   const int64_t *__restrict__ rpntr,
+                                                   const int64_t *__restrict__ cpntr,
+                                                   const int64_t *__restrict__ bpntrb,
+                                                   const int64_t *__restrict__ bpntre,
+                                                   const uint8_t *__restrict__ sparse, const int32_t *__restrict__ rmap,
+                                                   int64_t nrows, int64_t ncols, int64_t *__restrict__ cnt,
+                                                   const int64_t *__restrict__ rowptr, int64_t *__restrict__ cols,
+                                                   double *__restrict__ vals, int32_t *__restrict__ lastcol)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int64_t a = rmap[r];
+    int64_t n = 0, pos = cols ? rowptr[r] : 0;
+    if (bpntrb[a] >= 0) {
+        const int64_t h = rpntr[a + 1] - rpntr[a], lr = r - rpntr[a];
+        for (int64_t k = bpntrb[a]; k < bpntre[a]; ++k) {
+            if (!sparse[k]) continue;
+            const int64_t b = bindx[k], c0 = cpntr[b], w = cpntr[b + 1] - c0;
+            const double *__restrict__ blk = val + indx[k] + lr;
+            for (int64_t c = 0; c < w; ++c) {
+                const double x = blk[c * h];
+                if (x != 0.0) {
+                    if (cols) {
+                        cols[pos] = c0 + c;
+                        vals[pos] = x;
+                        ++pos;
+                    } else {
+                        ++n;
+                        if (c0 + c == ncols - 1) *lastcol = 1;
+                    }
+                }
+            }
+        }
+    }
+    if (!cols) cnt[r] = n;
+}

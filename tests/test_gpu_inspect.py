@@ -90,3 +90,45 @@ def test_bad_inputs_rejected():
     for rp, cp in (([0, 3, 5], [0, 6]), ([0, 3, 3, 6], [0, 6]), ([1, 6], [0, 6])):
         with pytest.raises(SableError):
             C.sparse_to_vbr_device(A, rp, cp)
+
+
+def test_compressed_kat_threshold_80(kat, tmp_path):
+    from sable_amd import fileio
+    d = np.array(kat["dense_test_compression"], dtype=float)
+    rp, cp = kat["rpntr"], kat["cpntr"]
+    val, indx, bindx, bb, be = C.convert_sparse_to_vbr(scipy.sparse.csc_matrix(d), rp, cp, None, None, device=True)
+    print("case: compressed KAT T=80", flush=True)
+    out = C.convert_vbr_to_compressed(val, rp, cp, indx, bindx, bb, be, "example2", str(tmp_path), 80, device=True)
+    val2, indx2, bindx2, bb2, be2, ub, indptr, indices, csr_val = out
+    ex = kat["compressed_threshold_80"]
+    assert _eq(val2, ex["val2"]) and _eq(indx2, ex["indx2"]) and _eq(ub, ex["ublocks"])
+    assert _eq(indptr, ex["indptr"]) and _eq(indices, ex["indices"]) and _eq(csr_val, ex["csr_val"])
+    r = fileio.read_vbrc(str(tmp_path / "example2.vbrc"))
+    assert _eq(r[8], ex["indptr"]) and _eq(r[9], ex["indices"]) and _eq(r[10], ex["csr_val"])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compress_matches_host(seed):
+    """Random routes on random VBRs (empty block rows, all-zero sparse
+    blocks, the last row / column with and without entries): the device
+    arrays equal the host's, dtypes included."""
+    rng = np.random.default_rng(seed)
+    vbr, _ = synth.vbr_matrix_gen_seeded(300, 260, 30, 25, 200, seed=seed + 40,
+                                         perc_zeros=int(rng.integers(0, 95)))
+    v = np.asarray(vbr.val, dtype=np.float64 if seed % 2 else np.float32)
+    v[rng.random(len(v)) < rng.uniform(0, 0.95)] = 0.0
+    nb = len(vbr.bindx)
+    for routes in (rng.integers(0, 2, nb).astype(np.uint8), np.ones(nb, np.uint8), np.zeros(nb, np.uint8)):
+        print(f"case: compress seed {seed} sparse blocks {int(routes.sum())}", flush=True)
+        a = (v, vbr.rpntr, vbr.cpntr, vbr.indx, vbr.bindx, vbr.bpntrb, vbr.bpntre, routes)
+        host = C.compress_with_routes(*a)
+        dev = C.compress_with_routes_device(*a)
+        for h, d in zip(host, dev):
+            assert h.dtype == d.dtype and np.array_equal(h, d)
+
+
+def test_compress_rejects_bad_vbr():
+    from sable_amd._lib import SableError
+    v = np.ones(8)
+    with pytest.raises(SableError):       # indx step does not match the block size
+        C.compress_with_routes_device(v, [0, 2], [0, 2, 4], [0, 3, 8], [0, 1], [0], [2], [1, 1])
