@@ -349,12 +349,13 @@ def bench_partition(args, runtime, info):
 
 
 def bench_inspect(args, runtime, info):
-    """Inspector workload (utils/convert_real_to_vbr.py:98-199,
-    convert_sparse_to_vbr): the 100,000 x 100,000 SpMV matrix as CSC (its
-    51 M stored nonzeros, FP64) and its 1000 x 1000 partitions to the VBR
-    arrays on the GPU.  Time: the device pipeline after the input upload
-    (sable_inspect_create); the arrays are checked against the host
-    restatement bit for bit.  CPU baseline: the oracle's per-entry
+    """Inspector workload (utils/convert_real_to_vbr.py:18-199,
+    convert_sparse_to_vbr then convert_vbr_to_compressed at T = 15): the
+    100,000 x 100,000 SpMV matrix as CSC (its 51 M stored nonzeros, FP64)
+    and its 1000 x 1000 partitions to the VBR arrays, the routes (from the
+    device's per-block nonzero counts) and the compressed arrays on the GPU.
+    Time: the two device pipelines after their input uploads; every array is
+    checked against the host restatement bit for bit.  CPU baseline: the oracle's per-entry
     restatement on the first 30 x 30 blocks, as entries per second."""
     from sable_amd import synth
     from sable_amd import convert as CV
@@ -368,15 +369,32 @@ def bench_inspect(args, runtime, info):
         out = CV.sparse_to_vbr_device(A, rp, cp)
         if rep >= args.warmup:
             times.append(out[-1])
-    val, indx, bindx, bb, be, _, _ = out
+    val, indx, bindx, bb, be, bnnz, _ = out
+    # routing (T = 15, the reference's strict > on the density percentage)
+    # from the device's per-block counts, then the compressed arrays
+    sizes = np.diff(indx)
+    routes = np.where(CV.block_density_percent(bnnz, sizes) > args.threshold, CV.ROUTE_DENSE,
+                      CV.ROUTE_SPARSE).astype(np.uint8)
+    ctimes = []
+    for rep in range(args.warmup + args.steps):
+        comp, cms = CV.compress_with_routes_device(val, rp, cp, indx, bindx, bb, be, routes, return_ms=True)
+        if rep >= args.warmup:
+            ctimes.append(cms)
     t0 = time.time()
     host = CV.convert_sparse_to_vbr(A, rp, cp, None, None, allow_empty_rows=True)
+    host_routes = CV.threshold_routes(host[0], host[1], args.threshold)
+    host_comp = CV.compress_with_routes(host[0], rp, cp, host[1], host[2], host[3], host[4], host_routes)
     host_s = time.time() - t0
     assert all(np.array_equal(a, b) for a, b in zip(host, (val, indx, bindx, bb, be))), "device != host"
-    ms = float(np.mean(times))
-    bytes_alg = 12 * A.nnz + 8 * (A.shape[1] + 1) + 8 * len(val) + 16 * len(bindx) + 16 * (len(rp) - 1)
+    assert np.array_equal(routes, host_routes), "routes differ"
+    assert all(np.array_equal(a, b) for a, b in zip(host_comp, comp)), "compressed arrays differ"
+    ms = float(np.mean(times)) + float(np.mean(ctimes))
+    # materialisation: entries read, val written; compression: val read,
+    # val2 and the CSR written
+    bytes_alg = (12 * A.nnz + 8 * (A.shape[1] + 1) + 8 * len(val) + 16 * len(bindx) + 16 * (len(rp) - 1)
+                 + 8 * len(val) + 8 * len(comp[0]) + 16 * len(comp[4]) + 8 * len(comp[3]))
     res = {
-        "metric": "inspector (convert_sparse_to_vbr): CSC + partitions to VBR arrays",
+        "metric": "inspector (convert_sparse_to_vbr + routing + convert_vbr_to_compressed): CSC to .vbrc arrays",
         "value": bytes_alg / (ms * 1e-3) / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64 (values), int64/int32 (indices)", "data": "synthetic",
@@ -385,6 +403,8 @@ def bench_inspect(args, runtime, info):
         "roofline": {"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": bytes_alg / (ms * 1e-3) / 1e9 / 8000.0, "traffic": None},
         "bytes_alg": bytes_alg, "entries_per_s": A.nnz / (ms * 1e-3), "host_restatement_s": host_s,
+        "materialise_ms": float(np.mean(times)), "compress_ms": float(np.mean(ctimes)),
+        "sparse_blocks": int(np.sum(routes == CV.ROUTE_SPARSE)), "csr_nnz": int(len(comp[4])),
         "matches_host": True, "cpu_baseline": None, "device": info,
     }
     if not args.no_cpu:
