@@ -429,7 +429,7 @@ def spmv_options(args):
     """Plan options: the autotuned table's winners unless given on the command line."""
     from sable_amd.autotune import tuned_options
     opts = {k: v for k, v in tuned_options("spmv").items() if k in ("mode", "tile_bytes", "unroll", "wg")}
-    for k in ("mode", "tile_bytes", "unroll"):
+    for k in ("mode", "tile_bytes", "unroll", "wg"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
     return opts
@@ -448,6 +448,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=None, help="default: the autotuned value")
     ap.add_argument("--tile-bytes", type=int, default=None, help="default: the autotuned value")
+    ap.add_argument("--wg", type=int, default=None, help="SpMV workgroup size; default: the autotuned value")
     ap.add_argument("--unroll", type=int, default=None, help="default: the autotuned value")
     ap.add_argument("--threshold", type=float, default=15)
     args = ap.parse_args()

@@ -224,8 +224,8 @@ bool collect_sparse_rows(const SableVbr &v, const std::vector<ColSeg> &g, int32_
 // minimise padded slots plus half the slots the busiest sparse wavefront
 // keeps the tile waiting for (a slice start counts two steps).  Returns the
 // cost.
-double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::vector<SparseRows> &sp,
-                        const std::vector<std::vector<ColSeg>> &sgroups, int32_t hs, int nsw)
+// The tile's rows by total sparse length, longest first (ties by row).
+std::vector<int32_t> sparse_row_order(const std::vector<SparseRows> &sp, int32_t hs)
 {
     std::vector<int64_t> tot(hs, 0);
     for (auto &sr : sp)
@@ -233,40 +233,64 @@ double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::ve
     std::vector<int32_t> order(hs);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return tot[x] > tot[y]; });
+    return order;
+}
 
-    // steps of every (piece, slice) for lanes-per-row 2^glog
-    auto widths = [&](int glog, std::vector<std::vector<int32_t>> *out) {
-        const int G = 1 << glog, C = 64 >> glog;
-        const int nsl = (hs + C - 1) / C;
-        int64_t padded = 0, crit = 0;
-        std::vector<int64_t> wsteps(nsw, 0);
-        for (size_t p = 0; p < sp.size(); ++p) {
-            std::vector<int32_t> w(nsl, 0);
-            for (int s = 0; s < nsl; ++s) {
-                int32_t m = 0;
-                for (int q = 0; q < C && s * C + q < hs; ++q) {
-                    const int32_t r = order[s * C + q];
-                    const int32_t len = sp[p].rp[r + 1] - sp[p].rp[r];
-                    m = std::max(m, (len + 4 * G - 1) / (4 * G));
-                }
-                w[s] = m;
-                wsteps[s % nsw] += m + 2;
-                padded += 256LL * m;
+// Steps of every (piece, slice) for lanes-per-row 2^glog and nsw sparse
+// wavefronts: returns {padded slots, steps of the busiest wavefront, a slice
+// start counting start_steps}.
+std::pair<int64_t, int64_t> slice_widths(const std::vector<SparseRows> &sp, const std::vector<int32_t> &order,
+                                         int32_t hs, int glog, int nsw, std::vector<std::vector<int32_t>> *out,
+                                         int start_steps = 2)
+{
+    const int G = 1 << glog, C = 64 >> glog;
+    const int nsl = (hs + C - 1) / C;
+    int64_t padded = 0, crit = 0;
+    std::vector<int64_t> wsteps(nsw, 0);
+    for (size_t p = 0; p < sp.size(); ++p) {
+        std::vector<int32_t> w(nsl, 0);
+        for (int s = 0; s < nsl; ++s) {
+            int32_t m = 0;
+            for (int q = 0; q < C && s * C + q < hs; ++q) {
+                const int32_t r = order[s * C + q];
+                const int32_t len = sp[p].rp[r + 1] - sp[p].rp[r];
+                m = std::max(m, (len + 4 * G - 1) / (4 * G));
             }
-            if (out) out->push_back(w);
+            w[s] = m;
+            wsteps[s % nsw] += m + start_steps;
+            padded += 256LL * m;
         }
-        for (int k = 0; k < nsw; ++k) crit = std::max(crit, wsteps[k]);
-        return std::make_pair(padded, crit);
-    };
+        if (out) out->push_back(w);
+    }
+    for (int k = 0; k < nsw; ++k) crit = std::max(crit, wsteps[k]);
+    return std::make_pair(padded, crit);
+}
+
+// G minimising padded slots plus half the slots the busiest sparse wavefront
+// keeps the tile waiting for; *crit = that wavefront's steps.
+int best_slice_glog(const std::vector<SparseRows> &sp, const std::vector<int32_t> &order, int32_t hs, int nsw,
+                    int64_t *crit)
+{
     int best = 0;
     double best_cost = 0;
+    int64_t best_crit = 0;
     for (int glog = 0; glog <= 6; ++glog) {
-        const auto pc = widths(glog, nullptr);
+        const auto pc = slice_widths(sp, order, hs, glog, nsw, nullptr);
         const double c = (double)pc.first + 0.5 * 256.0 * nsw * (double)pc.second;
-        if (glog == 0 || c < best_cost) { best = glog; best_cost = c; }
+        if (glog == 0 || c < best_cost) { best = glog; best_cost = c; best_crit = pc.second; }
     }
+    if (crit) *crit = best_crit;
+    return best;
+}
+
+// glog < 0: G by best_slice_glog.
+double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::vector<SparseRows> &sp,
+                        const std::vector<std::vector<ColSeg>> &sgroups, int32_t hs, int nsw, int glog)
+{
+    const std::vector<int32_t> order = sparse_row_order(sp, hs);
+    const int best = glog >= 0 ? glog : best_slice_glog(sp, order, hs, nsw, nullptr);
     std::vector<std::vector<int32_t>> W;
-    widths(best, &W);
+    slice_widths(sp, order, hs, best, nsw, &W);
     const int G = 1 << best, C = 64 >> best;
     const int nsl = (hs + C - 1) / C;
     t.sp_glog = best;
@@ -432,28 +456,52 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             t.y_accum = y_accum;
             // sparse rows of the segment, per sparse column group
             std::vector<SparseRows> sp;
-            int64_t sp_nnz = 0;
             for (size_t gi = 0; use_vec && gi < sgroups.size(); ++gi) {
                 SparseRows sr;
                 if (collect_sparse_rows(v, sgroups[gi], h, s0, hs, sr)) {
                     sr.group = (int32_t)gi;
-                    sp_nnz += (int64_t)sr.val.size();
                     sp.push_back(std::move(sr));
                 }
             }
             // SpMV with dense and sparse pieces: the last sp_waves wavefronts
-            // take the (latency-bound) sparse pieces while the others stream
-            // the dense ones, in proportion to the pieces' bytes (at least
-            // one each; measured best against heavier sparse weightings);
-            // the dense stream needs at least hs lanes.
+            // take the sparse slices while the others stream the dense
+            // pieces.  Both sides are chains of dependent loads, so the split
+            // k and the sparse lanes per row G are chosen together for the
+            // shortest longest chain, in round trips: a dense lane's float4
+            // loads / (unroll * batches in flight) against the busiest sparse
+            // wavefront's steps / (SB_SPUNROLL * batches in flight) plus two
+            // per slice (its table entry, then its first batch); among
+            // choices within 5 % of it, the one with the fewest padded
+            // sparse slots.  The dense stream needs at least hs lanes.
             const int nw = o.wg / 64;
             t.sp_waves = 0;
+            int sp_glog = -1;
             if (!spmm && !sp.empty() && use_vec && !dgroups.empty() && nw >= 2) {
-                const double sb = 6.0 * (double)sp_nnz, db = 4.0 * hs * (double)wd;
-                int k = (int)std::lround(nw * sb / (sb + db));
-                k = std::max(1, std::min(k, nw - 1));
-                while (k > 0 && 64 * (nw - k) < hs) --k;
-                t.sp_waves = k;
+                int64_t dq = 0;
+                for (auto &g : dgroups) dq += ((int64_t)hs * group_width(g) + 3) / 4;
+                const std::vector<int32_t> order = sparse_row_order(sp, hs);
+                struct Choice { int k, glog; double chain; int64_t padded; };
+                std::vector<Choice> cs;
+                for (int k = 1; k < nw && 64 * (nw - k) >= hs; ++k) {
+                    int32_t tp = 0, mc = 0;
+                    dense_stream_params(std::min<int32_t>(hs, 64 * (nw - k)), 64 * (nw - k), tp, mc);
+                    const double dch = std::ceil((double)dq / std::max(tp, 1)) /
+                                       (std::max(o.unroll, 1) * SB_SPMV_INFLIGHT);
+                    constexpr int per_rt = SB_SPUNROLL * SB_SPMV_INFLIGHT;   // sparse steps per round trip
+                    for (int g = 0; g <= 6; ++g) {
+                        const auto pw = slice_widths(sp, order, hs, g, k, nullptr, 2 * per_rt);
+                        cs.push_back({k, g, std::max(dch, (double)pw.second / per_rt), pw.first});
+                    }
+                }
+                if (!cs.empty()) {
+                    double best = cs[0].chain;
+                    for (auto &c : cs) best = std::min(best, c.chain);
+                    const Choice *pick = nullptr;
+                    for (auto &c : cs)
+                        if (c.chain <= 1.05 * best && (!pick || c.padded < pick->padded)) pick = &c;
+                    t.sp_waves = pick->k;
+                    sp_glog = pick->glog;
+                }
             }
             const int dlanes = 64 * (nw - t.sp_waves);
             dense_stream_params(std::min<int32_t>(hs, dlanes), dlanes, t.dense_tpar, t.dense_mcols);
@@ -532,7 +580,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                         cost += 6.0 * tot + 4.0 * (hs + 1);
                     }
                 } else {
-                    cost += emit_wave_slices(P, S, t, sp, sgroups, hs, t.sp_waves ? t.sp_waves : nw);
+                    cost += emit_wave_slices(P, S, t, sp, sgroups, hs, t.sp_waves ? t.sp_waves : nw, sp_glog);
                 }
             }
             t.piece_end = (int32_t)P.pieces.size();
@@ -541,6 +589,11 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                 std::stable_partition(P.pieces.begin() + t.piece_begin, P.pieces.begin() + t.piece_end,
                                       [](const SbPiece &pc) { return pc.kind == SB_PIECE_SPARSE; });
                 stage_groups(P, t);
+                // the first group's map, copied into the tile
+                if (t.piece_end > t.piece_begin) {
+                    t.stage_map = P.pieces[t.piece_begin].xmap_off;
+                    t.stage_len = P.pieces[t.piece_begin].stage_len;
+                }
             }
             if (spmm) cost += 2.0 * hs * P.n_rhs;   // Y write share
             P.tiles.push_back(t);

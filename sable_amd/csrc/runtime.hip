@@ -914,6 +914,7 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
             SbTile tt = p->host.tiles[t];
             tt.piece_begin += pbase;
             tt.piece_end += pbase;
+            if (tt.stage_len > 0) tt.stage_map += (int32_t)xmap_n;
             tt.mat = (int32_t)m;
             if (tt.sp_map >= 0) tt.sp_map += (int32_t)srow_n;
             tiles.push_back(tt);

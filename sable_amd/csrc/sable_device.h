@@ -35,7 +35,16 @@ struct SbTile {
                              // last ones; the rest stream the dense pieces), or 0
                              // when every wavefront does both in turn
     int32_t y_accum;         // SpMM: 1 = add to Y rows an MFMA tile wrote before
+    int32_t stage_map;       // SpMV: xmap_off and stage_len of the first staging group
+    int32_t stage_len;       // (copies, so its map loads follow the tile read directly)
 };
+
+// SpMV sparse slices: steps per pipelined batch; and the batches the SpMV
+// stream loops (dense and sparse) complete per round trip (their register
+// copies of the next batch wait for it).  The plan builder balances the
+// sparse and dense wavefronts' dependent chains with them.
+#define SB_SPUNROLL 2
+#define SB_SPMV_INFLIGHT 1
 
 // LDS for the staged X rows of a FP32 SpMM piece (64 KiB): the plan builder
 // gives such a piece at most SB_SPMM_XS_FLOATS / N columns.

@@ -152,12 +152,8 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
 // Row r's result is (dense partial sums of the M column groups, in group
 // order) + (sparse sums, piece by piece): deterministic, no atomics.
 
-// Sparse steps per pipelined batch, and the waves per SIMD the SpMV kernel
-// is compiled for (its VGPR budget) given the dense unroll: 80 VGPRs (6
-// waves) at unroll 2, 96 (5 waves) at unroll 4.
-#ifndef SB_SPUNROLL
-#define SB_SPUNROLL 2
-#endif
+// The waves per SIMD the SpMV kernel is compiled for (its VGPR budget) given
+// the dense unroll: 80 VGPRs (6 waves) at unroll 2, 96 (5 waves) at unroll 4.
 #define SB_SPMV_WAVES(U) ((U) <= 2 ? 6 : (U) <= 4 ? 5 : 1)
 
 struct SbStreamLane {
@@ -205,6 +201,12 @@ __device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int 
     if (tid >= tpar || tid >= nq) return;
     const int nmine = (nq - 1 - tid) / tpar + 1;             // float4 loads of this lane
     const int nbatch = (nmine + UNROLL - 1) / UNROLL;
+    // The register copy cur = nxt waits for the batch it copies, so one
+    // batch is in flight per round trip.  A copy-free form (two buffers that
+    // trade roles) keeps two in flight but needs about 113 VGPRs (4 waves
+    // per SIMD, two 512-thread workgroups per CU) and measured no faster on
+    // the 100k matrix: the kernel runs at about 85 % of a cold streaming
+    // read of the same bytes either way.
     sb_f4 cur[UNROLL], nxt[UNROLL];
     sb_dense_load<UNROLL>(P, nq, tid, tpar, cur);
     for (int b = 0; b < nbatch; ++b) {
@@ -327,16 +329,22 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
     const int nsw = tile.sp_waves > 0 ? tile.sp_waves : NW;
     const int sw = wave - (NW - nsw);                     // sparse wavefront index, < 0: none
     const int nsl = (hs + (64 >> tile.sp_glog) - 1) / (64 >> tile.sp_glog);   // sparse slices
-
     int pb = tile.piece_begin;
+    bool head = true;
     while (pb < tile.piece_end) {
-        // staging group [pb, pe), formed by the host (stage_len <= XCAP + 4 slots)
+        // staging group [pb, pe), formed by the host (stage_len <= XCAP + 4
+        // slots); the first group's map offset and length come with the
+        // tile, so its map loads need not wait for the piece descriptor
         const SbPiece first = a.pieces[pb];
-        const int pe = first.grp_end;
-        if (pe <= pb || first.stage_len > XCAP + 4) break;   // never built (the host and kernel choice check)
+        const int smap = head ? tile.stage_map : first.xmap_off;
+        const int slen = head ? tile.stage_len : first.stage_len;
+        head = false;
+        if (slen > XCAP + 4) break;            // never built (the host and kernel choice check)
         __syncthreads();                       // previous group done with xs
-        sb_stage_map<WG>(a.xmap + first.xmap_off, first.stage_len, x, xs);
+        sb_stage_map<WG>(a.xmap + smap, slen, x, xs);
         __syncthreads();
+        const int pe = first.grp_end;
+        if (pe <= pb) break;                   // never built
         int base = 0;
         for (int p = pb; p < pe; ++p) {
             const SbPiece pc = a.pieces[p];

@@ -176,6 +176,9 @@ def check_staging(plan):
     assert plan.stats["xmap_entries"] == len(xmap)
     for t in tiles:
         p = int(t["piece_begin"])
+        # the first group's map, copied into the tile
+        if t["piece_end"] > t["piece_begin"]:
+            assert (t["stage_map"], t["stage_len"]) == (pieces[p]["xmap_off"], pieces[p]["stage_len"])
         while p < t["piece_end"]:
             q = int(pieces[p]["grp_end"])
             assert p < q <= t["piece_end"]
