@@ -210,7 +210,7 @@ int sable_rtc_precompile(int kind, int wg, int unroll, int xcap, int nrhs, int r
 
 /* Achievable HBM rates (no reference counterpart): best of `reps` launches
  * of a streaming read and a streaming copy over `bytes` (GB/s; the copy
- * counts read + write bytes). */
+ * counts read + write bytes), each launch cold (after a 1 GiB overwrite). */
 int sable_bandwidth_probe(int64_t bytes, int reps, float *read_gbs, float *copy_gbs);
 
 /* Block partitioner, one axis (src/autopartition.py:34-68 cut_indices2 with
