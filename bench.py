@@ -17,7 +17,7 @@ per launch / measured kernel time vs 8 TB/s HBM), both plans' figures, and
 the reference CPU path (the restated generated-C program, FP64 like the
 reference) timed on this host's cores on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5|partition|inspect] [--warm]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5|partition|inspect|io] [--warm]
                     [--routing plain|learned|both] [--no-cpu]
 """
 from __future__ import annotations
@@ -348,6 +348,97 @@ def bench_partition(args, runtime, info):
     print(json.dumps(out), flush=True)
 
 
+def bench_io(args, runtime, info):
+    """Text and binary I/O of the 100,000 x 100,000 SpMV matrix (the .vbr
+    text of utils/fileio.py:13-64, about 1 GB): the native writer, the native
+    parallel reader (arrays checked bit for bit against what was written),
+    the binary container (save, load by memory map, load + upload of the
+    arrays to HBM).  The file sits in the page cache after the write, so the
+    reads measure parsing, not the disk.  CPU baseline: the reference's parse
+    (Python float() / int() per token) on the first 2 M values of the val
+    line, scaled to the file."""
+    import shutil
+    from sable_amd import synth, fileio
+    from sable_amd.vbr import VBR
+    from oracle.fileio_oracle import parse_line_reference
+    vbr, _, _ = synth.workload_c2()
+    v = VBR(np.asarray(vbr.val, dtype=np.float64), *(np.asarray(getattr(vbr, n), dtype=np.int64) for n in
+                                                    ("indx", "bindx", "rpntr", "cpntr", "bpntrb", "bpntre")))
+    d = tempfile.mkdtemp(prefix="sable_io_")
+    try:
+        cores = len(os.sched_getaffinity(0))
+        t0 = time.time()
+        fileio.write_vbr_matrix("c2", v, d)
+        write_s = time.time() - t0
+        path = os.path.join(d, "c2.vbr")
+        nbytes = os.path.getsize(path)
+        reads = []
+        for rep in range(max(1, args.steps)):
+            t0 = time.time()
+            arrs = fileio.read_vbr(path)
+            reads.append(time.time() - t0)
+        names = ("val", "indx", "bindx", "rpntr", "cpntr", "bpntrb", "bpntre")
+        assert all(np.array_equal(a.view(np.uint64) if a.dtype == np.float64 else a,
+                                  np.asarray(getattr(v, n)).view(np.uint64) if n == "val" else np.asarray(getattr(v, n)))
+                   for a, n in zip(arrs, names)), "text round trip differs"
+        read_s = min(reads)
+        log(f"io: {nbytes / 1e9:.3f} GB of text; write {write_s:.2f} s, read {read_s:.3f} s on {cores} threads")
+        bpath = os.path.join(d, "c2.bin")
+        t0 = time.time()
+        fileio.save_vbr_bin(bpath, v)
+        bsave_s = time.time() - t0
+        bbytes = os.path.getsize(bpath)
+        t0 = time.time()
+        w = fileio.load_vbr_bin(bpath, mmap=False)
+        bload_s = time.time() - t0
+        assert all(np.array_equal(np.asarray(getattr(w, n)), np.asarray(getattr(v, n))) for n in names)
+        t0 = time.time()
+        w = fileio.load_vbr_bin(bpath, mmap=True)
+        dv = runtime.DeviceArray.from_numpy(np.asarray(w.val, dtype=np.float32))
+        bhbm_s = time.time() - t0
+        del dv
+        # the reference's parse on a bounded sample of the val line
+        with open(path) as f:
+            line = f.readline()
+        cut = 0
+        for _ in range(2_000_000):
+            cut = line.index(",", cut) + 1
+        sample = line[:cut - 1] + "]\n"
+        t0 = time.time()
+        ref_vals = parse_line_reference(sample, float)
+        ref_s = time.time() - t0
+        assert np.array_equal(np.asarray(ref_vals).view(np.uint64), arrs[0][:len(ref_vals)].view(np.uint64))
+        ref_rate = len(sample) / ref_s
+        # and its writer (str() per value, utils/fileio.py:28) on the same values
+        t0 = time.time()
+        ref_text = ",".join(map(str, ref_vals))
+        ref_write_s = time.time() - t0
+        assert ref_text == sample[sample.index("[") + 1:-2]
+        cpu = {"value": ref_rate / 1e9, "unit": "GB/s of text", "cores": 1, "kind": "port",
+               "sample": f"the reference's read_vbr parse (Python float() per token, utils/fileio.py:55-64) on the "
+                         f"first {len(ref_vals)} values of the val line ({len(sample) / 1e6:.1f} MB, {ref_s:.1f} s)",
+               "est_full_file_s": nbytes / ref_rate,
+               "write_sample_s": ref_write_s, "est_full_write_s": ref_write_s * nbytes / len(sample)}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    line = {
+        "metric": "VBR text I/O: .vbr parse throughput of the 100k matrix (native parallel reader)",
+        "value": nbytes / read_s / 1e9, "unit": "GB/s of text", "n_gpus": 1, "steps": max(1, args.steps),
+        "warmup": 0, "ms_per_step": read_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 text", "data": "synthetic",
+        "config": {"workload": "100,000 x 100,000 VBR, 1000 x 1000 partitions, 51 M stored values, .vbr text "
+                               f"({nbytes / 1e9:.2f} GB) in the page cache"},
+        "text_bytes": nbytes, "threads": cores, "write_s": write_s, "read_s": read_s,
+        "binary_bytes": bbytes, "binary_save_s": bsave_s, "binary_load_s": bload_s,
+        "binary_load_gbs": bbytes / bload_s / 1e9,
+        "binary_load_and_upload_val_s": bhbm_s,
+        "cpu_baseline": cpu, "speedup_vs_reference_parse": (nbytes / ref_rate) / read_s,
+        "device": info,
+    }
+    print(json.dumps(line))
+    return line
+
+
 def bench_inspect(args, runtime, info):
     """Inspector workload (utils/convert_real_to_vbr.py:18-199,
     convert_sparse_to_vbr then convert_vbr_to_compressed at T = 15): the
@@ -440,7 +531,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "partition", "inspect"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "partition", "inspect", "io"])
     ap.add_argument("--count", type=int, default=256, help="c5: number of matrices")
     ap.add_argument("--warm", action="store_true", help="no cache flush between launches")
     ap.add_argument("--routing", default="both", choices=["plain", "learned", "both"],
@@ -479,6 +570,8 @@ def main():
         return bench_partition(args, runtime, info)
     elif args.workload == "inspect":
         return bench_inspect(args, runtime, info)
+    elif args.workload == "io":
+        return bench_io(args, runtime, info)
     else:
         return bench_spmm(args, runtime, info)
     log(f"workload generated in {time.time() - t0:.1f}s: {len(vbr.val)} stored values, {len(vbr.bindx)} blocks")

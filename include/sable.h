@@ -55,6 +55,7 @@ typedef struct SablePlan SablePlan;
 typedef struct SableGroup SableGroup;
 typedef struct SableInspect SableInspect;
 typedef struct SableCompress SableCompress;
+typedef struct SableText SableText;
 
 /* The VBR arrays of src/vbr.py:7-14 (validated: see sable_plan_create). */
 typedef struct {
@@ -267,6 +268,28 @@ int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, 
 int sable_compress_fetch(SableCompress *h, double *val2, int64_t *indx2, int64_t *ublocks, int64_t *indptr,
                          int64_t *indices, double *csr_val);
 int sable_compress_destroy(SableCompress *h);
+
+/* Text `.vbr` / `.vbrc` lines `name=[a,b,...]` (host only, no device).
+ * sable_text_open + sable_text_read_* replace the readers of
+ * utils/fileio.py:55-64 (read_vbr) and :66-93 (read_vbrc): the file is
+ * memory-mapped, counts[i] receives the element count of line i (0 for
+ * "[]"), and each line is parsed by nthreads threads (0: the CPUs this
+ * process may run on) with correctly rounded decimal conversion, so the
+ * values equal Python's float() / int() of each token bit for bit.
+ * sable_text_write replaces the writers of utils/fileio.py:13-35
+ * (write_vbr_matrix) and utils/convert_real_to_vbr.py:84-95 (.vbrc): one line
+ * per array, doubles formatted as Python's str() formats them, so a file
+ * round-trips text-identically.  Errors of these four functions are read
+ * with sable_text_last_error. */
+#define SABLE_TEXT_F64 0
+#define SABLE_TEXT_I64 1
+int sable_text_open(const char *path, int nlines, int nthreads, SableText **out, int64_t *counts);
+int sable_text_read_f64(SableText *h, int line, double *out);
+int sable_text_read_i64(SableText *h, int line, int64_t *out);
+void sable_text_close(SableText *h);
+int sable_text_write(const char *path, int nlines, const char *const *names, const int32_t *kinds,
+                     const void *const *arrays, const int64_t *counts, int append, int nthreads);
+const char *sable_text_last_error(void);
 
 /* Device memory helpers for callers without another allocator. */
 int sable_malloc(void **ptr, int64_t bytes);

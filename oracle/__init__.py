@@ -14,6 +14,8 @@ Contents
                 (CSR part first, then dense blocks in ordinal order with the
                 three kernel shapes; SpMM with the RHS width as a parameter),
                 built by ``oracle/Makefile`` into ``oracle/_build/``.
+``fileio_oracle``  the reference's text parse (Python float()/int() per
+                token), the baseline and checker of the native text reader.
 ``ref_build``   recipe that compiles the reference's own ``src/csr-spmv.cpp``
                 from ``/root/reference`` into ``oracle/_ref/`` when present.
 

@@ -118,6 +118,13 @@ _SIGS = {
                                     C.POINTER(I64), C.POINTER(I64), F32P]),
     "sable_compress_fetch": (I32, [P, P, P, P, P, P, P]),
     "sable_compress_destroy": (I32, [P]),
+    "sable_text_open": (I32, [C.c_char_p, I32, I32, C.POINTER(P), C.POINTER(I64)]),
+    "sable_text_read_f64": (I32, [P, I32, P]),
+    "sable_text_read_i64": (I32, [P, I32, P]),
+    "sable_text_close": (None, [P]),
+    "sable_text_write": (I32, [C.c_char_p, I32, C.POINTER(C.c_char_p), C.POINTER(I32), C.POINTER(P),
+                               C.POINTER(I64), I32, I32]),
+    "sable_text_last_error": (C.c_char_p, []),
     "sable_malloc": (I32, [C.POINTER(P), I64]),
     "sable_free": (I32, [P]),
     "sable_memcpy_h2d": (I32, [P, P, I64]),
