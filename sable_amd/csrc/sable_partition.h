@@ -55,12 +55,16 @@ __device__ __forceinline__ double sb_shift_dot(const int32_t *__restrict__ ix, c
 }
 
 // One wavefront per adjacent pair (four pairs per 256-thread workgroup).
+// When line b has at most SB_SIM_STAGE entries its row indices are staged in
+// LDS first (one coalesced pass, together with its nonzero count), so the
+// searches below read LDS; longer lines are searched in global memory.
 // Per chunk of 64 entries of line a, lane l takes entry k = chunk + l (a
 // coalesced read), binary-searches row ra - 1 in line b and checks the next
 // three entries, forming its products for d0 (rb = ra), d1 (rb = ra - 1) and
 // d2 (rb = ra + 1).  Lanes 0-2 then add the chunk's products of d0 / d1 / d2
 // from LDS in entry order, so every sum is the sequential ascending-row sum
 // of the scalar merge.  Nonzero counts are wavefront sums of 0/1 flags.
+#define SB_SIM_STAGE 2048
 __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict__ ip,
                                                        const int32_t *__restrict__ ix,
                                                        const double *__restrict__ v, int64_t nlines,
@@ -68,18 +72,28 @@ __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict
 {
 #pragma clang fp contract(off)
     __shared__ double prod[4][3][64];
+    __shared__ int32_t brow[4][SB_SIM_STAGE];
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     const int64_t i = (int64_t)blockIdx.x * 4 + wave;
     if (i + 1 >= nlines) return;                      // the whole wavefront
     const int64_t a0 = ip[i], a1 = ip[i + 1], b0 = a1, b1 = ip[i + 2];
+    const bool staged = b1 - b0 <= SB_SIM_STAGE;      // wave-uniform
+    int32_t *bs = brow[wave];
     int ca = 0, cb = 0;
     for (int64_t k = a0 + lane; k < a1; k += 64) ca += v[k] != 0.0;
-    for (int64_t k = b0 + lane; k < b1; k += 64) cb += v[k] != 0.0;
+    for (int64_t k = b0 + lane; k < b1; k += 64) {
+        cb += v[k] != 0.0;
+        if (staged) bs[k - b0] = ix[k];
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         ca += __shfl_xor(ca, o);
         cb += __shfl_xor(cb, o);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nb = (int)(b1 - b0);
     double acc = 0.0;                                 // lane 0: d0, lane 1: d1, lane 2: d2
     for (int64_t c0 = a0; c0 < a1; c0 += 64) {
         const int64_t k = c0 + lane;
@@ -87,19 +101,43 @@ __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict
         if (k < a1) {
             const int32_t ra = ix[k];
             const double va = v[k];
-            int64_t lo = b0, hi = b1;
-            while (lo < hi) {                          // first entry of b with row >= ra - 1
-                const int64_t mid = (lo + hi) >> 1;
-                if (ix[mid] < ra - 1) lo = mid + 1;
-                else hi = mid;
-            }
-            for (int q = 0; q < 3 && lo + q < b1; ++q) {
-                const int32_t rb = ix[lo + q];
-                if (rb > ra + 1) break;
-                const double pr = va * v[lo + q];
-                if (rb == ra) p0 = pr;
-                else if (rb == ra - 1) p1 = pr;
-                else p2 = pr;
+            if (staged) {
+                int lo = 0, hi = nb;
+                while (lo < hi) {                      // first entry of b with row >= ra - 1
+                    const int mid = (lo + hi) >> 1;
+                    if (bs[mid] < ra - 1) lo = mid + 1;
+                    else hi = mid;
+                }
+                int32_t rb[3];
+                double vb[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {          // independent loads, issued together
+                    const bool in = lo + q < nb;
+                    rb[q] = in ? bs[lo + q] : INT32_MAX;
+                    vb[q] = in ? v[b0 + lo + q] : 0.0;
+                }
+                for (int q = 0; q < 3; ++q) {
+                    if (rb[q] > ra + 1) break;
+                    const double pr = va * vb[q];
+                    if (rb[q] == ra) p0 = pr;
+                    else if (rb[q] == ra - 1) p1 = pr;
+                    else p2 = pr;
+                }
+            } else {
+                int64_t lo = b0, hi = b1;
+                while (lo < hi) {                      // first entry of b with row >= ra - 1
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (ix[mid] < ra - 1) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (int q = 0; q < 3 && lo + q < b1; ++q) {
+                    const int32_t rb = ix[lo + q];
+                    if (rb > ra + 1) break;
+                    const double pr = va * v[lo + q];
+                    if (rb == ra) p0 = pr;
+                    else if (rb == ra - 1) p1 = pr;
+                    else p2 = pr;
+                }
             }
         }
         prod[wave][0][lane] = p0;
