@@ -182,18 +182,39 @@ __global__ void __launch_bounds__(256) sb_scatter_values(const int32_t *__restri
                                                          double *__restrict__ val, int32_t *__restrict__ bnnz)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    const int64_t i = ix[k], j = col[k];
-    const int64_t a = rmap[i], b = cmap[j];
-    const int64_t ord = sb_block_ordinal(bits, wpre, rowbeg, rw, a, b);
-    const int64_t h = rpntr[a + 1] - rpntr[a];
-    val[indx[ord] + (j - cpntr[b]) * h + (i - rpntr[a])] = v[k];
-    // the first entry of a run (same column, same block row) counts the
-    // run's nonzero values and adds them once
-    if (k > 0 && col[k - 1] == j && rmap[ix[k - 1]] == a) return;
-    int32_t nzr = 0;
-    for (int64_t q = k; q < nnz && col[q] == j && rmap[ix[q]] == a; ++q) nzr += v[q] != 0.0;
-    if (nzr) atomicAdd(&bnnz[ord], nzr);
+    const int lane = (int)(threadIdx.x & 63);
+    const bool live = k < nnz;                        // live lanes are a prefix of the wavefront
+    int64_t ord = -1;
+    int32_t a = -1, j = -1, cnt = 0;
+    if (live) {
+        const int32_t i = ix[k];
+        j = col[k];
+        a = rmap[i];
+        const int32_t b = cmap[j];
+        ord = sb_block_ordinal(bits, wpre, rowbeg, rw, a, b);
+        const int64_t h = rpntr[a + 1] - rpntr[a];
+        const double x = v[k];
+        val[indx[ord] + (j - cpntr[b]) * h + (i - rpntr[a])] = x;
+        cnt = x != 0.0;
+    }
+    // Nonzero count per run (consecutive entries of one column in one block
+    // row, i.e. of one block): runs are contiguous, so a segmented suffix sum
+    // over the wavefront's lanes gives each run's count at its first lane in
+    // this wavefront, which adds it once.  A run that crosses a wavefront
+    // boundary is added in parts.
+    const int32_t jp = __shfl_up(j, 1), ap = __shfl_up(a, 1);
+    const bool head = lane == 0 || !live || jp != j || ap != a;
+    const unsigned long long starts = __ballot(head);
+    const unsigned long long upto = lane == 63 ? starts : starts & ((2ull << lane) - 1ull);
+    const int seg = 63 - __builtin_clzll(upto);        // this lane's run start (lane 0 is always one)
+    int32_t sum = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t s2 = __shfl_down(sum, o);
+        const int g2 = __shfl_down(seg, o);
+        if (lane + o < 64 && g2 == seg) sum += s2;
+    }
+    if (live && lane == seg && sum) atomicAdd(&bnnz[ord], sum);
 }
 
 // ---- compressed (.vbrc) arrays on the device (reference:
@@ -243,16 +264,22 @@ __global__ void __launch_bounds__(256) sb_csr_rows(const double *__restrict__ va
             if (!sparse[k]) continue;
             const int64_t b = bindx[k], c0 = cpntr[b], w = cpntr[b + 1] - c0;
             const double *__restrict__ blk = val + indx[k] + lr;
-            for (int64_t c = 0; c < w; ++c) {
-                const double x = blk[c * h];
-                if (x != 0.0) {
+            // eight columns per step, their loads issued together (a
+            // column past the block reads its last column again)
+            for (int64_t c = 0; c < w; c += 8) {
+                double xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) xv[u] = blk[(c + u < w ? c + u : w - 1) * h];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (c + u >= w || xv[u] == 0.0) continue;
                     if (cols) {
-                        cols[pos] = c0 + c;
-                        vals[pos] = x;
+                        cols[pos] = c0 + c + u;
+                        vals[pos] = xv[u];
                         ++pos;
                     } else {
                         ++n;
-                        if (c0 + c == ncols - 1) *lastcol = 1;
+                        if (c0 + c + u == ncols - 1) *lastcol = 1;
                     }
                 }
             }
