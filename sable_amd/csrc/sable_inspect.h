@@ -236,7 +236,15 @@ __global__ void __launch_bounds__(256) sb_gather_dense(const double *__restrict_
     const int64_t d = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (d >= ndense) return;
     const int64_t k = dblk[d], src = indx[k], n = indx[k + 1] - src, dst = indx2[d];
-    for (int64_t e = threadIdx.x & 63; e < n; e += 64) val2[dst + e] = val[src + e];
+    // four loads in flight per lane, then their stores
+    for (int64_t e = threadIdx.x & 63; e < n; e += 256) {
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = e + 64 * u < n ? val[src + e + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e + 64 * u < n) val2[dst + e + 64 * u] = x[u];
+    }
 }
 
 // per row: the nonzero values of its sparse-routed blocks (count pass when
