@@ -59,9 +59,9 @@ __device__ __forceinline__ double sb_shift_dot(const int32_t *__restrict__ ix, c
 // LDS first (one coalesced pass, together with its nonzero count), so the
 // searches below read LDS; longer lines are searched in global memory.
 // Per chunk of 64 entries of line a, lane l takes entry k = chunk + l (a
-// coalesced read), binary-searches row ra - 1 in line b and checks the next
-// three entries, forming its products for d0 (rb = ra), d1 (rb = ra - 1) and
-// d2 (rb = ra + 1).  Lanes 0-2 then add the chunk's products of d0 / d1 / d2
+// coalesced read, requested one chunk ahead), binary-searches row ra - 1 in
+// line b and checks the next three entries, forming its products for d0
+// (rb = ra), d1 (rb = ra - 1) and d2 (rb = ra + 1).  Lanes 0-2 then add the chunk's products of d0 / d1 / d2
 // from LDS in entry order, so every sum is the sequential ascending-row sum
 // of the scalar merge.  Nonzero counts are wavefront sums of 0/1 flags.
 #define SB_SIM_STAGE 2048
@@ -95,12 +95,22 @@ __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int nb = (int)(b1 - b0);
     double acc = 0.0;                                 // lane 0: d0, lane 1: d1, lane 2: d2
+    // this chunk's entry of line a, loaded one chunk ahead (indices past the
+    // line read its last entry; such lanes form no product)
+    int32_t ra = 0;
+    double va = 0.0;
+    if (a1 > a0) {
+        const int64_t k0 = a0 + lane < a1 ? a0 + lane : a1 - 1;
+        ra = ix[k0];
+        va = v[k0];
+    }
     for (int64_t c0 = a0; c0 < a1; c0 += 64) {
         const int64_t k = c0 + lane;
+        const int64_t kn = k + 64 < a1 ? k + 64 : a1 - 1;
+        const int32_t ra_next = ix[kn];
+        const double va_next = v[kn];
         double p0 = 0.0, p1 = 0.0, p2 = 0.0;
         if (k < a1) {
-            const int32_t ra = ix[k];
-            const double va = v[k];
             if (staged) {
                 int lo = 0, hi = nb;
                 while (lo < hi) {                      // first entry of b with row >= ra - 1
@@ -153,6 +163,8 @@ __global__ void __launch_bounds__(256) sb_adjacent_sim(const int64_t *__restrict
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        ra = ra_next;
+        va = va_next;
     }
     const double d0 = __shfl(acc, 0), d1 = __shfl(acc, 1), d2 = __shfl(acc, 2);
     if (lane != 0) return;
