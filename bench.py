@@ -479,6 +479,17 @@ def bench_inspect(args, runtime, info):
     assert all(np.array_equal(a, b) for a, b in zip(host, (val, indx, bindx, bb, be))), "device != host"
     assert np.array_equal(routes, host_routes), "routes differ"
     assert all(np.array_equal(a, b) for a, b in zip(host_comp, comp)), "compressed arrays differ"
+    # end to end (uploads and downloads included): two calls, val going to
+    # the host and back in between, against one call keeping it on the device
+    t0 = time.time()
+    v_, ix_, bx_, bb_, be_, _, _ = CV.sparse_to_vbr_device(A, rp, cp)
+    CV.compress_with_routes_device(v_, rp, cp, ix_, bx_, bb_, be_, routes)
+    two_call_s = time.time() - t0
+    t0 = time.time()
+    one = CV.sparse_to_vbrc_device(A, rp, cp, density=args.threshold)
+    one_call_s = time.time() - t0
+    assert all(np.array_equal(a, b) for a, b in zip(one, (comp[0], comp[1], bindx, bb, be, comp[2], comp[3],
+                                                        comp[4], comp[5]))), "one-call arrays differ"
     ms = float(np.mean(times)) + float(np.mean(ctimes))
     # materialisation: entries read, val written; compression: val read,
     # val2 and the CSR written
@@ -494,6 +505,7 @@ def bench_inspect(args, runtime, info):
         "roofline": {"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": bytes_alg / (ms * 1e-3) / 1e9 / 8000.0, "traffic": None},
         "bytes_alg": bytes_alg, "entries_per_s": A.nnz / (ms * 1e-3), "host_restatement_s": host_s,
+        "end_to_end_two_calls_s": two_call_s, "end_to_end_one_call_s": one_call_s,
         "materialise_ms": float(np.mean(times)), "compress_ms": float(np.mean(ctimes)),
         "sparse_blocks": int(np.sum(routes == CV.ROUTE_SPARSE)), "csr_nnz": int(len(comp[4])),
         "matches_host": True, "cpu_baseline": None, "device": info,

@@ -268,6 +268,13 @@ int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, 
 int sable_compress_fetch(SableCompress *h, double *val2, int64_t *indx2, int64_t *ublocks, int64_t *indptr,
                          int64_t *indices, double *csr_val);
 int sable_compress_destroy(SableCompress *h);
+/* Compression straight from an inspector handle: convert_sparse_to_vbr then
+ * convert_vbr_to_compressed (utils/convert_real_to_vbr.py:98-199, :18-96)
+ * without the VBR arrays leaving the device.  routes has one entry per block
+ * of h (as sable_compress_create's); the result is read with
+ * sable_compress_fetch and released with sable_compress_destroy. */
+int sable_inspect_compress(SableInspect *h, const uint8_t *routes, SableCompress **out, int64_t *nval2,
+                           int64_t *nublocks, int64_t *csr_nnz, float *device_ms);
 
 /* Text `.vbr` / `.vbrc` lines `name=[a,b,...]` (host only, no device).
  * sable_text_open + sable_text_read_* replace the readers of

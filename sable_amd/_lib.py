@@ -118,6 +118,7 @@ _SIGS = {
                                     C.POINTER(I64), C.POINTER(I64), F32P]),
     "sable_compress_fetch": (I32, [P, P, P, P, P, P, P]),
     "sable_compress_destroy": (I32, [P]),
+    "sable_inspect_compress": (I32, [P, P, C.POINTER(P), C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), F32P]),
     "sable_text_open": (I32, [C.c_char_p, I32, I32, C.POINTER(P), C.POINTER(I64)]),
     "sable_text_read_f64": (I32, [P, I32, P]),
     "sable_text_read_i64": (I32, [P, I32, P]),

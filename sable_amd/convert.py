@@ -131,15 +131,22 @@ def compress_with_routes_device(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, 
                                     len(rp) - 1, cp.ctypes.data, len(cp) - 1, bb.ctypes.data, be.ctypes.data,
                                     rt.ctypes.data, C.byref(h), C.byref(nv2), C.byref(nub), C.byref(nnz),
                                     C.byref(ms)))
+    out = _fetch_compressed(h, nb, nv2.value, nub.value, nnz.value, int(rp[-1]), int(cp[-1]), val.dtype)
+    return (out, ms.value) if return_ms else out
+
+
+def _fetch_compressed(h, nb, nv2, nub, nnz, nrows, ncols, val_dtype):
+    """The arrays of a compress handle (released here), in compress_with_routes'
+    order and dtypes."""
+    from ._lib import lib, check
     try:
-        val2 = np.empty(nv2.value, dtype=np.float64)
-        indx2 = np.empty(nb + 1 - nub.value, dtype=np.int64)
-        ub = np.empty(nub.value, dtype=np.int64)
-        nrows = int(rp[-1])
-        if nnz.value >= 0:
+        val2 = np.empty(nv2, dtype=np.float64)
+        indx2 = np.empty(nb + 1 - nub, dtype=np.int64)
+        ub = np.empty(nub, dtype=np.int64)
+        if nnz >= 0:
             ip = np.empty(nrows + 1, dtype=np.int64)
-            ind = np.empty(nnz.value, dtype=np.int64)
-            cv = np.empty(nnz.value, dtype=np.float64)
+            ind = np.empty(nnz, dtype=np.int64)
+            cv = np.empty(nnz, dtype=np.float64)
             check(lib.sable_compress_fetch(h, val2.ctypes.data, indx2.ctypes.data, ub.ctypes.data, ip.ctypes.data,
                                            ind.ctypes.data, cv.ctypes.data))
         else:
@@ -147,19 +154,59 @@ def compress_with_routes_device(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, 
                                            None))
     finally:
         lib.sable_compress_destroy(h)
-    val2 = val2.astype(val.dtype)
-    if nnz.value < 0:
-        out = (val2, indx2, ub, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.float64))
-    else:
-        # the index dtypes scipy gives the host construction (COO of int64
-        # row / column arrays -> CSR), and the float64 the padding entry
-        # brings in
-        probe = scipy.sparse.coo_array((np.zeros(1), (np.array([nrows - 1], dtype=np.int64),
-                                                      np.array([int(cp[-1]) - 1], dtype=np.int64)))).tocsr()
-        padded = cv[-1] == 0.0
-        out = (val2, indx2, ub, ip.astype(probe.indptr.dtype), ind.astype(probe.indices.dtype),
-               cv if padded else cv.astype(val.dtype))
-    return (out, ms.value) if return_ms else out
+    val2 = val2.astype(val_dtype)
+    if nnz < 0:
+        return val2, indx2, ub, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.float64)
+    # the index dtypes scipy gives the host construction (COO of int64 row /
+    # column arrays -> CSR), and the float64 the padding entry brings in
+    probe = scipy.sparse.coo_array((np.zeros(1), (np.array([nrows - 1], dtype=np.int64),
+                                                  np.array([ncols - 1], dtype=np.int64)))).tocsr()
+    padded = cv[-1] == 0.0
+    return (val2, indx2, ub, ip.astype(probe.indptr.dtype), ind.astype(probe.indices.dtype),
+            cv if padded else cv.astype(val_dtype))
+
+
+def sparse_to_vbrc_device(A, rpntr, cpntr, density=15, return_ms=False):
+    """convert_sparse_to_vbr then convert_vbr_to_compressed with the fixed
+    threshold (utils/convert_real_to_vbr.py:98-199, :18-96), the VBR arrays
+    staying on the GPU between the two (sable_inspect_create, then
+    sable_inspect_compress): returns (val2, indx2, bindx, bpntrb, bpntre,
+    ublocks, indptr, indices, csr_val) as convert_vbr_to_compressed does, bit
+    for bit.  A is a CSC matrix with summed duplicates; empty block rows get
+    -1/-1.  Routes: dense iff the block's nonzero share in percent is
+    strictly above density."""
+    import ctypes as C
+    from . import runtime
+    from ._lib import lib, check
+    runtime.init()
+    rp = np.ascontiguousarray(rpntr, dtype=np.int64)
+    cp = np.ascontiguousarray(cpntr, dtype=np.int64)
+    ip = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(A.indices, dtype=np.int32)
+    v = np.ascontiguousarray(A.data, dtype=np.float64)
+    h, nb, nv, ms = C.c_void_p(), C.c_int64(), C.c_int64(), C.c_float()
+    check(lib.sable_inspect_create(ip.ctypes.data, ix.ctypes.data, v.ctypes.data, A.shape[0], A.shape[1],
+                                   rp.ctypes.data, len(rp) - 1, cp.ctypes.data, len(cp) - 1, C.byref(h),
+                                   C.byref(nb), C.byref(nv), C.byref(ms)))
+    try:
+        indx = np.empty(nb.value + 1, dtype=np.int64)
+        bindx = np.empty(nb.value, dtype=np.int64)
+        bb = np.empty(len(rp) - 1, dtype=np.int64)
+        be = np.empty(len(rp) - 1, dtype=np.int64)
+        bnnz = np.empty(nb.value, dtype=np.int32)
+        check(lib.sable_inspect_fetch(h, None, indx.ctypes.data, bindx.ctypes.data, bb.ctypes.data,
+                                      be.ctypes.data, bnnz.ctypes.data))
+        routes = np.where(block_density_percent(bnnz, np.diff(indx)) > density, ROUTE_DENSE,
+                          ROUTE_SPARSE).astype(np.uint8)
+        hc, nv2, nub, nnz, ms2 = C.c_void_p(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_float()
+        check(lib.sable_inspect_compress(h, routes.ctypes.data if len(routes) else None, C.byref(hc),
+                                         C.byref(nv2), C.byref(nub), C.byref(nnz), C.byref(ms2)))
+    finally:
+        lib.sable_inspect_destroy(h)
+    val2, indx2, ub, indptr, indices, csr_val = _fetch_compressed(hc, nb.value, nv2.value, nub.value, nnz.value,
+                                                                  int(rp[-1]), int(cp[-1]), v.dtype)
+    out = (val2, indx2, bindx, bb, be, ub, indptr, indices, csr_val)
+    return (out, ms.value + ms2.value) if return_ms else out
 
 
 def convert_vbr_to_compressed(val, rpntr, cpntr, indx, bindx, bpntrb, bpntre,

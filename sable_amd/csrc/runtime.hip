@@ -1344,6 +1344,7 @@ int sable_axis_cuts(const int64_t *indptr, const int32_t *indices, const double 
 struct SableInspect {
     int64_t nblocks = 0, nval = 0;
     std::vector<int64_t> bpntrb, bpntre;
+    std::vector<int64_t> rpntr, cpntr;   // the partitions it was built for
     double *d_val = nullptr;
     int64_t *d_indx = nullptr, *d_bindx = nullptr;
     int32_t *d_bnnz = nullptr;
@@ -1400,6 +1401,8 @@ int sable_inspect_create(const int64_t *indptr, const int32_t *indices, const do
     for (int64_t k = 0; k < nnz; ++k)
         if (indices[k] < 0 || indices[k] >= nrows) return fail(SABLE_ERR_INVALID, "row index out of range");
     std::unique_ptr<SableInspect> h(new SableInspect());
+    h->rpntr.assign(rpntr, rpntr + nbr + 1);
+    h->cpntr.assign(cpntr, cpntr + nbc + 1);
     DevBufs b;
     const int64_t rw = (nbc + 31) / 32;
     int64_t *d_ip = nullptr, *d_rp = nullptr, *d_cp = nullptr, *d_rowbeg = nullptr, *d_size = nullptr;
@@ -1530,35 +1533,20 @@ struct SableCompress {
     }
 };
 
-int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, const int64_t *bindx,
-                          int64_t nblocks, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
-                          const int64_t *bpntrb, const int64_t *bpntre, const uint8_t *routes,
-                          SableCompress **out, int64_t *nval2, int64_t *nublocks, int64_t *csr_nnz,
-                          float *device_ms)
+}  // extern "C"
+
+namespace {
+
+// Compression of VBR arrays already on the device (val, indx, bindx) for
+// the given routes; indx, the partitions and the block-row pointers on the
+// host.  Shared by sable_compress_create (after uploading) and
+// sable_inspect_compress (the inspector's own arrays).
+int compress_on_device(const double *d_val, const int64_t *d_indx, const int64_t *d_bindx, const int64_t *indx,
+                       int64_t nblocks, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
+                       const int64_t *bpntrb, const int64_t *bpntre, const uint8_t *routes, SableCompress **out,
+                       int64_t *nval2, int64_t *nublocks, int64_t *csr_nnz, float *device_ms)
 {
-    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
-    if (!out || !indx || !bpntrb || !bpntre || nblocks < 0 || nval < 0) return fail(SABLE_ERR_INVALID, "bad compress arguments");
-    *out = nullptr;
-    if (nblocks > 0 && (!val || !bindx || !routes)) return fail(SABLE_ERR_INVALID, "null val/bindx/routes");
-    const int64_t nrows = rpntr ? rpntr[nbr] : 0, ncols = cpntr ? cpntr[nbc] : 0;
-    CHECKRC(check_partition(rpntr, nbr, nrows, "rpntr"));
-    CHECKRC(check_partition(cpntr, nbc, ncols, "cpntr"));
-    // the kernels read block k only inside [indx[k], indx[k] + h w): check it
-    if (indx[0] != 0 || indx[nblocks] != nval) return fail(SABLE_ERR_INVALID, "indx must run from 0 to len(val)");
-    int64_t next = 0;
-    for (int32_t a = 0; a < nbr; ++a) {
-        if (bpntrb[a] == -1 && bpntre[a] == -1) continue;
-        if (bpntrb[a] != next || bpntre[a] < bpntrb[a] || bpntre[a] > nblocks)
-            return fail(SABLE_ERR_INVALID, "bpntrb/bpntre must lay the block rows out in order");
-        const int64_t h = rpntr[a + 1] - rpntr[a];
-        for (int64_t k = bpntrb[a]; k < bpntre[a]; ++k) {
-            if (bindx[k] < 0 || bindx[k] >= nbc) return fail(SABLE_ERR_INVALID, "bindx out of range");
-            if (indx[k + 1] - indx[k] != h * (cpntr[bindx[k] + 1] - cpntr[bindx[k]]))
-                return fail(SABLE_ERR_INVALID, "indx step must equal the block size");
-        }
-        next = bpntre[a];
-    }
-    if (next != nblocks) return fail(SABLE_ERR_INVALID, "bpntre must end at the block count");
+    const int64_t nrows = rpntr[nbr], ncols = cpntr[nbc];
     std::unique_ptr<SableCompress> h(new SableCompress());
     h->nrows = nrows;
     std::vector<int64_t> dblk;
@@ -1576,14 +1564,10 @@ int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, 
     h->nval2 = h->indx2.back();
     h->nub = (int64_t)h->ublocks.size();
     DevBufs b;
-    double *d_val = nullptr;
-    int64_t *d_indx = nullptr, *d_bindx = nullptr, *d_rp = nullptr, *d_cp = nullptr, *d_bb = nullptr,
-            *d_be = nullptr, *d_dblk = nullptr, *d_indx2 = nullptr, *d_cnt = nullptr;
+    int64_t *d_rp = nullptr, *d_cp = nullptr, *d_bb = nullptr, *d_be = nullptr, *d_dblk = nullptr,
+            *d_indx2 = nullptr, *d_cnt = nullptr;
     uint8_t *d_sp = nullptr;
     int32_t *d_rmap = nullptr, *d_last = nullptr;
-    CHECKRC(b.alloc((void **)&d_val, nval * sizeof(double)));
-    CHECKRC(b.alloc((void **)&d_indx, (nblocks + 1) * sizeof(int64_t)));
-    CHECKRC(b.alloc((void **)&d_bindx, nblocks * sizeof(int64_t)));
     CHECKRC(b.alloc((void **)&d_rp, (nbr + 1) * sizeof(int64_t)));
     CHECKRC(b.alloc((void **)&d_cp, (nbc + 1) * sizeof(int64_t)));
     CHECKRC(b.alloc((void **)&d_bb, nbr * sizeof(int64_t)));
@@ -1594,9 +1578,6 @@ int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, 
     CHECKRC(b.alloc((void **)&d_sp, sp.size()));
     CHECKRC(b.alloc((void **)&d_rmap, nrows * sizeof(int32_t)));
     CHECKRC(b.alloc((void **)&d_last, sizeof(int32_t)));
-    if (nval) HIPOK(hipMemcpy(d_val, val, nval * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(d_indx, indx, (nblocks + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-    if (nblocks) HIPOK(hipMemcpy(d_bindx, bindx, nblocks * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(d_rp, rpntr, (nbr + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(d_cp, cpntr, (nbc + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(d_bb, bpntrb, nbr * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -1663,6 +1644,52 @@ int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, 
     return SABLE_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int sable_compress_create(const double *val, int64_t nval, const int64_t *indx, const int64_t *bindx,
+                          int64_t nblocks, const int64_t *rpntr, int32_t nbr, const int64_t *cpntr, int32_t nbc,
+                          const int64_t *bpntrb, const int64_t *bpntre, const uint8_t *routes,
+                          SableCompress **out, int64_t *nval2, int64_t *nublocks, int64_t *csr_nnz,
+                          float *device_ms)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (!out || !indx || !bpntrb || !bpntre || nblocks < 0 || nval < 0) return fail(SABLE_ERR_INVALID, "bad compress arguments");
+    *out = nullptr;
+    if (nblocks > 0 && (!val || !bindx || !routes)) return fail(SABLE_ERR_INVALID, "null val/bindx/routes");
+    const int64_t nrows = rpntr ? rpntr[nbr] : 0, ncols = cpntr ? cpntr[nbc] : 0;
+    CHECKRC(check_partition(rpntr, nbr, nrows, "rpntr"));
+    CHECKRC(check_partition(cpntr, nbc, ncols, "cpntr"));
+    // the kernels read block k only inside [indx[k], indx[k] + h w): check it
+    if (indx[0] != 0 || indx[nblocks] != nval) return fail(SABLE_ERR_INVALID, "indx must run from 0 to len(val)");
+    int64_t next = 0;
+    for (int32_t a = 0; a < nbr; ++a) {
+        if (bpntrb[a] == -1 && bpntre[a] == -1) continue;
+        if (bpntrb[a] != next || bpntre[a] < bpntrb[a] || bpntre[a] > nblocks)
+            return fail(SABLE_ERR_INVALID, "bpntrb/bpntre must lay the block rows out in order");
+        const int64_t h = rpntr[a + 1] - rpntr[a];
+        for (int64_t k = bpntrb[a]; k < bpntre[a]; ++k) {
+            if (bindx[k] < 0 || bindx[k] >= nbc) return fail(SABLE_ERR_INVALID, "bindx out of range");
+            if (indx[k + 1] - indx[k] != h * (cpntr[bindx[k] + 1] - cpntr[bindx[k]]))
+                return fail(SABLE_ERR_INVALID, "indx step must equal the block size");
+        }
+        next = bpntre[a];
+    }
+    if (next != nblocks) return fail(SABLE_ERR_INVALID, "bpntre must end at the block count");
+    DevBufs b;
+    double *d_val = nullptr;
+    int64_t *d_indx = nullptr, *d_bindx = nullptr;
+    CHECKRC(b.alloc((void **)&d_val, nval * sizeof(double)));
+    CHECKRC(b.alloc((void **)&d_indx, (nblocks + 1) * sizeof(int64_t)));
+    CHECKRC(b.alloc((void **)&d_bindx, nblocks * sizeof(int64_t)));
+    if (nval) HIPOK(hipMemcpy(d_val, val, nval * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(d_indx, indx, (nblocks + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (nblocks) HIPOK(hipMemcpy(d_bindx, bindx, nblocks * sizeof(int64_t), hipMemcpyHostToDevice));
+    return compress_on_device(d_val, d_indx, d_bindx, indx, nblocks, rpntr, nbr, cpntr, nbc, bpntrb, bpntre, routes,
+                              out, nval2, nublocks, csr_nnz, device_ms);
+}
+
 int sable_compress_fetch(SableCompress *h, double *val2, int64_t *indx2, int64_t *ublocks, int64_t *indptr,
                          int64_t *indices, double *csr_val)
 {
@@ -1682,6 +1709,20 @@ int sable_compress_destroy(SableCompress *h)
 {
     delete h;
     return SABLE_OK;
+}
+
+int sable_inspect_compress(SableInspect *ih, const uint8_t *routes, SableCompress **out, int64_t *nval2,
+                           int64_t *nublocks, int64_t *csr_nnz, float *device_ms)
+{
+    if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    if (!ih || !out || (ih->nblocks > 0 && !routes)) return fail(SABLE_ERR_INVALID, "bad inspect_compress arguments");
+    *out = nullptr;
+    std::vector<int64_t> indx(ih->nblocks + 1);
+    HIPOK(hipMemcpy(indx.data(), ih->d_indx, indx.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return compress_on_device(ih->d_val, ih->d_indx, ih->d_bindx, indx.data(), ih->nblocks, ih->rpntr.data(),
+                              (int32_t)ih->rpntr.size() - 1, ih->cpntr.data(), (int32_t)ih->cpntr.size() - 1,
+                              ih->bpntrb.data(), ih->bpntre.data(), routes, out, nval2, nublocks, csr_nnz,
+                              device_ms);
 }
 
 // ---- device memory helpers

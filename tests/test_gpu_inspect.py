@@ -132,3 +132,37 @@ def test_compress_rejects_bad_vbr():
     v = np.ones(8)
     with pytest.raises(SableError):       # indx step does not match the block size
         C.compress_with_routes_device(v, [0, 2], [0, 2, 4], [0, 3, 8], [0, 1], [0], [2], [1, 1])
+
+
+def test_sparse_to_vbrc_on_device_kat_and_host():
+    """CSC -> .vbrc arrays with the VBR arrays kept on the device between the
+    two steps (sable_inspect_compress): the reference's T = 80 known answer,
+    and on random matrices (empty block rows, explicit zeros, several
+    thresholds) the host pipeline's arrays, dtypes included."""
+    import json, os
+    kat = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat_example.json")))
+    d = np.array(kat["dense_test_compression"], dtype=float)
+    print("case: sparse_to_vbrc KAT T=80", flush=True)
+    val2, indx2, bindx, bb, be, ub, indptr, indices, csr_val = C.sparse_to_vbrc_device(
+        scipy.sparse.csc_matrix(d), kat["rpntr"], kat["cpntr"], density=80)
+    ex = kat["compressed_threshold_80"]
+    assert _eq(val2, ex["val2"]) and _eq(indx2, ex["indx2"]) and _eq(ub, ex["ublocks"])
+    assert _eq(indptr, ex["indptr"]) and _eq(indices, ex["indices"]) and _eq(csr_val, ex["csr_val"])
+    rng = np.random.default_rng(31)
+    for t in range(4):
+        n, m = int(rng.integers(200, 600)), int(rng.integers(200, 600))
+        d = rng.standard_normal((n, m))
+        d[rng.random((n, m)) < rng.uniform(0.5, 0.99)] = 0.0
+        d[:, :7] = 0.0                            # empty block rows / columns somewhere
+        A = scipy.sparse.csc_matrix(d)
+        rp = np.unique(np.concatenate([[0, n], rng.integers(1, n, 20)]))
+        cp = np.unique(np.concatenate([[0, m], rng.integers(1, m, 18)]))
+        for T in (0, 15, 80):
+            print(f"case: sparse_to_vbrc random #{t} T={T}", flush=True)
+            dev = C.sparse_to_vbrc_device(A, rp, cp, density=T)
+            val, indx, bindx, bb, be = C.convert_sparse_to_vbr(A, rp, cp, None, None, allow_empty_rows=True)
+            routes = C.threshold_routes(val, indx, T)
+            hv2, hix2, hub, hip, hind, hcv = C.compress_with_routes(val, rp, cp, indx, bindx, bb, be, routes)
+            host = (hv2, hix2, bindx, bb, be, hub, hip, hind, hcv)
+            for h, g in zip(host, dev):
+                assert np.asarray(h).dtype == np.asarray(g).dtype and np.array_equal(h, g)
