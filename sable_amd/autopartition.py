@@ -17,8 +17,8 @@ Instead of densifying two lines per step (the reference's per-line
 ``toarray()``), the three dot products of every adjacent pair are computed at
 once with sparse elementwise products (each summed in ascending index
 order), and only the final scan loops in Python.  ``device=True`` runs both
-on the GPU (``sable_axis_cuts``: one thread per adjacent pair, and the scan
-in its closed form per position), with the same result bit for bit.  A
+on the GPU (``sable_axis_cuts``: one wavefront per adjacent pair, and the
+scan in its closed form per position), with the same result bit for bit.  A
 user-supplied similarity function falls back to the per-pair loop.
 """
 from __future__ import annotations
@@ -202,20 +202,23 @@ def cut_indices1(A, cut_threshold, similarity):
 
 
 def my_convert_dense_to_vbrc(file_info, cut_threshold, cut_indices, similarity, *, router=None,
-                             allow_empty_rows=False):
+                             allow_empty_rows=False, device=False):
     """src/autopartition.py:196-205: .mtx -> partition -> .vbr -> routed
     .vbrc.  Returns (val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, ublocks,
-    indptr, indices, csr_val) like the reference."""
+    indptr, indices, csr_val) like the reference.  device=True runs the
+    partition (when cut_indices is cut_indices2), the block materialisation
+    and the compression on the GPU, with the same results."""
     src_path, dest_path = file_info
     mtx = scipy.io.mmread(src_path)
     A = scipy.sparse.csc_matrix(mtx, copy=False)
     del mtx
     gc.collect()
-    cpntr, rpntr = cut_indices(A, cut_threshold, similarity)
+    kw = {"device": True} if device and cut_indices is cut_indices2 else {}
+    cpntr, rpntr = cut_indices(A, cut_threshold, similarity, **kw)
     stem = pathlib.Path(src_path).resolve().stem
     dst = pathlib.Path(dest_path).resolve().parent
     val, indx, bindx, bpntrb, bpntre = convert_sparse_to_vbr(A, rpntr, cpntr, stem, dst,
-                                                             allow_empty_rows=allow_empty_rows)
+                                                             allow_empty_rows=allow_empty_rows, device=device)
     val, indx, bindx, bpntrb, bpntre, ublocks, indptr, indices, csr_val = convert_vbr_to_compressed(
-        val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, stem, dst, router=router)
+        val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, stem, dst, router=router, device=device)
     return val, rpntr, cpntr, indx, bindx, bpntrb, bpntre, ublocks, indptr, indices, csr_val

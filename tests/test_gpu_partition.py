@@ -90,3 +90,20 @@ def test_franz8_matches_reference_algorithm(golden):
     cols, rows = AP.cut_indices2(A, 0.2, AP.similarity2, device=True)
     rcols, rrows = O.cut_indices2_dense_lines(A, 0.2)
     assert cols == rcols and rows == rrows
+
+
+def test_dense_to_vbrc_pipeline_on_device(golden, tmp_path):
+    """my_convert_dense_to_vbrc (partition, materialisation, compression) with
+    device=True returns what the host pipeline returns, on Franz8."""
+    src = tmp_path / "Franz8.mtx"
+    with gzip.open(os.path.join(golden, "Franz8.mtx.gz"), "rt") as f:
+        src.write_text(f.read())
+    print("case: Franz8 pipeline", flush=True)
+    outs = []
+    for dev in (False, True):
+        d = tmp_path / ("dev" if dev else "host")
+        d.mkdir()
+        outs.append(AP.my_convert_dense_to_vbrc((str(src), str(d / "Franz8.vbrc")), 0.2, AP.cut_indices2,
+                                                AP.similarity2, allow_empty_rows=True, device=dev))
+    for h, g in zip(*outs):
+        assert np.array_equal(np.asarray(h), np.asarray(g))
