@@ -139,7 +139,13 @@ def bench_spmm(args, runtime, info):
         roof = {"bound": "mfma", "achieved": mfma_flops / (ms * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s (bf16, 6 passes of the 3-term split)"}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    # HBM bytes per launch of the kept variant, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload
     roof["traffic"] = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_spmm_c3.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            roof["traffic"] = json.load(f).get(kept, {}).get("hbm_bytes_per_launch")
     cpu = None
     if not args.no_cpu:
         Yc, ns = c_oracle.spmm(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre,
