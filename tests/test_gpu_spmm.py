@@ -190,3 +190,17 @@ def test_split_accuracy_bounds(rt):
     print(f"  errors {e}", flush=True)
     assert e[3] <= TOL and e[3] <= 4 * e["fp32"] + 1e-7
     assert e[2] <= 3 * 2.0 ** -16
+
+
+@pytest.mark.parametrize("n_rhs", [16, 32])
+def test_small_workgroups_all_routes(rt, n_rhs):
+    """128-thread workgroups (the tuned choice for N = 16): MFMA, FP32-dense
+    and sparse blocks in the same block rows, against the FP64 oracle."""
+    vbr = _c3_like(2500, seed=23)
+    rng = np.random.default_rng(24)
+    routes = rng.choice([ROUTE_DENSE, ROUTE_SPARSE, ROUTE_DENSE_MFMA], len(vbr.bindx)).astype(np.uint8)
+    X = rng.uniform(-1, 1, (vbr.cpntr[-1], n_rhs)).astype(np.float32)
+    print(f"case: wg 128 n={n_rhs}", flush=True)
+    plan = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs, wg=128)
+    assert plan.stats["wg"] == 128
+    assert _err(vbr, X, plan.spmm(X))["componentwise"] <= TOL
