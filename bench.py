@@ -210,10 +210,11 @@ def bench_batch(args, runtime, info):
             lr = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
                                       vbr.bpntre, k) if router else fixed)
             # built for the group: no per-plan minimum tile count (the group
-            # fills the GPU; small matrices keep block-row-sized tiles); the
-            # N = 16 SpMM members in 128-thread workgroups (the group measured
-            # 3.46 ms with them, 3.65 with 256, 4.56 with 512)
-            pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1, **({"wg": 128} if op == "spmm" else {}))
+            # fills the GPU; small matrices keep block-row-sized tiles), and
+            # smaller workgroups than the 100k tuning for these short tiles:
+            # SpMV 256 threads, SpMM N = 16 128 threads (the group measured
+            # 3.10 ms so, 3.46 with SpMV at 512, 3.65 with SpMM at 256 too)
+            pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1, wg=128 if op == "spmm" else 256)
             plain.append(pp)
             learned.append(pl)
             xs.append(runtime.DeviceArray.from_numpy(inp))

@@ -208,3 +208,16 @@ def test_tile_clock_reports_unsupported(rt):
     with pytest.raises(SableError):
         plan.enable_tile_clock(True)
     assert _err(vbr, x, plan.spmv(x))["componentwise"] <= TOL
+
+
+def test_mixed_routes_256_threads_unroll_2(rt):
+    """256-thread workgroups at unroll 2 (the 256-matrix batch's SpMV
+    members): dense and sparse pieces in the same tiles, split roles."""
+    vbr, x, _ = synth.workload_like_c2(6000, seed=17)
+    from sable_amd.convert import threshold_routes
+    routes = threshold_routes(vbr.val, vbr.indx, 15)
+    print("case: mixed wg=256 unroll=2", flush=True)
+    plan = rt.Plan.from_vbr(vbr, routes, wg=256, unroll=2, min_tiles=1)
+    assert plan.stats["wg"] == 256
+    assert any(int(t["sp_waves"]) > 0 for t in plan.export("tiles"))
+    assert _err(vbr, x, plan.spmv(x))["componentwise"] <= TOL
