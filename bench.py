@@ -8,10 +8,10 @@ matrix resident in HBM.  By default every step starts cold: a 1 GiB buffer is
 overwritten between launches (outside the timed events), so the 256 MB
 last-level cache holds none of the matrix.
 
-Two plans are measured: the plain one (fixed density threshold T = 15) and,
-when router weights are present, the learned one (the MLP router's routes);
-the learned plan is kept only if it is within 1e-5 of the FP64 oracle and
-faster.  Prints ONE JSON line: value = GFLOP/s (2 * nnz / time) of the kept
+Plans measured: the plain one (fixed density threshold T = 15) and, when
+router weights are present, the router's plans at sparse-route biases 0, 0.5
+and 1 (log2 ns); the fastest of those within 1e-5 of the FP64 oracle is the
+learned plan, kept only if it is faster than the plain one.  Prints ONE JSON line: value = GFLOP/s (2 * nnz / time) of the kept
 plan, the roofline of its kernel (algorithmic bytes of the executed format
 per launch / measured kernel time vs 8 TB/s HBM), both plans' figures, and
 the reference CPU path (the restated generated-C program, FP64 like the
@@ -86,7 +86,9 @@ def bench_spmm(args, runtime, info):
     2% blocks (70% density 0.9-1.0, 30% 0.02-0.1), N(0,1), X 50,000 x 128
     U[-1,1] (seed 3).  Plain = fixed threshold T=15 with dense blocks on the
     FP32 vector path; MFMA = the same routing with dense blocks on the bf16
-    split MFMA path (exact 3-term split, 6 passes; and hi+lo, 3 passes)."""
+    split MFMA path (exact 3-term split, 6 passes; and hi+lo, 3 passes);
+    learned = the fastest within 1e-5 of the router's plans at MFMA-route
+    biases 0, 0.5 and 1 (log2 ns), with the 3-term split."""
     from sable_amd import synth
     from sable_amd.convert import threshold_routes, ROUTE_DENSE, ROUTE_DENSE_MFMA
     from oracle import c_oracle
@@ -105,10 +107,13 @@ def bench_spmm(args, runtime, info):
                 ("mfma_x6", np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base), 3),
                 ("mfma_x3", np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base), 2)]
     from sable_amd.router import Router
+    from sable_amd.router.router import BIAS_GRID
     if args.routing in ("learned", "both") and Router.available():
-        lr = Router.load().route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                        vbr.bpntre, n_rhs)
-        variants.append(("learned_x6", lr, 3))
+        router = Router.load()
+        for sb, mb in BIAS_GRID["spmm"]:
+            lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                     vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
+            variants.append((f"learned_x6 mfma_bias={mb:g}", lr, 3))
     for name, routes, split in variants:
         t1 = time.time()
         plan = runtime.Plan.from_vbr(vbr, routes.astype(np.uint8), op="spmm", n_rhs=n_rhs, mfma_split=split)
@@ -120,7 +125,11 @@ def bench_spmm(args, runtime, info):
                      "err": err, "inspector_s": insp}
         log(f"{name}: {res[name]['ms']:.4f} ms, err {err['componentwise']:.2e}, rtc={st['rtc_used']}")
         plan.close()
-    # the learned plan is kept only if it is within tolerance and faster
+    # the fastest learned plan within tolerance is "learned_x6", kept only if faster
+    cands = [k for k in res if k.startswith("learned_x6 ")]
+    good = [k for k in cands if res[k]["err"]["componentwise"] <= 1e-5]
+    if cands:
+        res["learned_x6"] = res[min(good or cands, key=lambda k: res[k]["ms"])]
     kept = "mfma_x6"
     if "learned_x6" in res and res["learned_x6"]["err"]["componentwise"] <= 1e-5 and \
             res["learned_x6"]["ms"] < res["mfma_x6"]["ms"]:
@@ -163,8 +172,9 @@ def bench_spmm(args, runtime, info):
         "config": {"workload": "SpMM, 50000x50000 VBR, 500x500 partitions sized 32-192, 2% blocks, "
                                "70% density 0.9-1.0 / 30% 0.02-0.1, N(0,1), X 50000x128 U[-1,1], seed 3",
                    "routing": (f"{kept}: dense blocks on MFMA (3-term bf16 split, FP32 accumulate); "
-                               f"learned routes kept only if faster at 1e-5, else fixed threshold "
-                               f"T={args.threshold:g}"),
+                               f"learned = fastest of the router's plans at MFMA-route biases "
+                               f"{[mb for _, mb in BIAS_GRID['spmm']] if cands else []}, kept only if "
+                               f"faster at 1e-5, else fixed threshold T={args.threshold:g}"),
                    "cache": "warm" if args.warm else "cold (1 GiB flush between launches)"},
         "roofline": roof,
         "kept": kept,
@@ -174,7 +184,9 @@ def bench_spmm(args, runtime, info):
         "speedup_x3_vs_fp32_plain": res["fp32_plain"]["ms"] / res["mfma_x3"]["ms"],
         "variants": {k: {"ms": v["ms"], "median_ms": v["median_ms"], "err": v["err"],
                          "gflops": v["stats"]["flops_alg"] / (v["ms"] * 1e-3) / 1e9,
-                         "inspector_s": v["inspector_s"]} for k, v in res.items()},
+                         "inspector_s": v["inspector_s"], "bytes_alg": v["stats"]["bytes_alg"],
+                         "ntiles": v["stats"]["ntiles"], "ntiles_mfma": v["stats"]["ntiles_mfma"]}
+                     for k, v in res.items()},
         "bytes_alg": st["bytes_alg"], "device": info,
     }
     print(json.dumps(line), flush=True)
@@ -633,11 +645,21 @@ def main():
     if args.routing in ("learned", "both"):
         from sable_amd.router import Router
         if Router.available():
+            from sable_amd.router.router import BIAS_GRID
             router = Router.load()
-            lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                     vbr.bpntre, 1)
-            res["learned"] = measure(lr, "learned")
-            res["learned"]["routes_changed"] = int(np.sum(lr != routes))
+            # one learned plan per sparse-route bias; the fastest within 1e-5 is "learned"
+            cands = []
+            for sb, mb in BIAS_GRID["spmv"]:
+                lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                         vbr.bpntre, 1, sparse_bias=sb, mfma_bias=mb)
+                r = measure(lr, f"learned (sparse bias {sb:g})")
+                r["sparse_bias"] = sb
+                r["routes_changed"] = int(np.sum(lr != routes))
+                cands.append(r)
+            good = [r for r in cands if r["error_vs_fp64"]["componentwise"] <= 1e-5]
+            res["learned"] = min(good or cands, key=lambda r: r["ms"])
+            res["learned_candidates"] = [{k: r[k] for k in ("sparse_bias", "ms", "median_ms", "bytes_alg")}
+                                         for r in cands]
             ok = res["learned"]["error_vs_fp64"]["componentwise"] <= 1e-5
             # keep-if-faster at identical tolerance
             if args.routing == "learned" or (ok and res["learned"]["ms"] < res["plain"]["ms"]):
@@ -673,8 +695,9 @@ def main():
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc,
-                   "routing": (f"{kept} (learned kept only if faster at 1e-5; plain = fixed threshold "
-                               f"T={args.threshold:g})" if "learned" in res
+                   "routing": (f"{kept} (learned = fastest of the router's plans at sparse biases "
+                               f"{[c['sparse_bias'] for c in res['learned_candidates']]}, kept only if faster "
+                               f"at 1e-5; plain = fixed threshold T={args.threshold:g})" if "learned" in res
                                else f"plain fixed threshold T={args.threshold:g}"),
                    "cache": "warm" if args.warm else "cold (1 GiB flush between launches)",
                    "nrows": int(vbr.rpntr[-1]), "ncols": int(vbr.cpntr[-1]), "nnz": head["flops_alg"] // 2,

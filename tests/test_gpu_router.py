@@ -120,6 +120,29 @@ def test_choose_plan_keeps_the_faster_plan(mods):
     print("case: choose_plan on a 20k SpMV", flush=True)
     plan, rep = choose_plan(vbr, x, router=Router.load(), reps=10)
     assert rep["learned_within_tol"]
+    cands = rep["candidates"]
+    assert 1 <= len(cands) <= 3 and all(c["within_tol"] for c in cands)
+    assert rep["learned_ms"] == min(c["ms"] for c in cands)
     assert rep["kept"] == ("learned" if rep["learned_ms"] < rep["fixed_ms"] else "fixed")
     y = plan.spmv(x)
     assert np.all(np.isfinite(y))
+
+
+def test_route_biases(mods):
+    """A bias is subtracted from that route's predicted log2 cost: a large
+    sparse bias routes every block sparse, a large negative one none; a
+    larger bias never routes fewer blocks that way; the MFMA bias never
+    enables MFMA where N does not allow it."""
+    from sable_amd import synth
+    from sable_amd.convert import ROUTE_SPARSE, ROUTE_DENSE_MFMA
+    from sable_amd.router import Router
+    vbr, _, _ = synth.workload_like_c2(8000, seed=33)
+    r = Router.load()
+    args = (vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    print("case: route biases on an 8k matrix", flush=True)
+    base = r.route_blocks(*args, 1)
+    assert np.array_equal(base, r.route_blocks(*args, 1, sparse_bias=0.0, mfma_bias=0.0))
+    counts = [int(np.sum(r.route_blocks(*args, 1, sparse_bias=b) == ROUTE_SPARSE)) for b in (-100, 0, 0.5, 1, 100)]
+    assert counts[0] == 0 and counts[-1] == len(base) and counts == sorted(counts)
+    assert not np.any(r.route_blocks(*args, 1, mfma_bias=100) == ROUTE_DENSE_MFMA)
+    assert np.all(r.route_blocks(*args, 128, mfma_bias=100) == ROUTE_DENSE_MFMA)
