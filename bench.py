@@ -196,7 +196,8 @@ def bench_batch(args, runtime, info):
     """Batched workload: 256 VBR matrices (n log-uniform 2,000-64,000, P
     uniform 50-800 partitions sized [ceil(n/5P), floor(2n/P)], 1% blocks, the
     100k workload's density mix, seed 5), each with an SpMV and an SpMM with
-    N = 16.  Batched = learned routes (MFMA allowed for the SpMM), autotuned
+    N = 16.  Batched = learned routes (MFMA allowed for the SpMM; router
+    biases (0, 0), (0, 1) and (1, 1), the fastest group kept), autotuned
     tiles sized for the group (no per-plan minimum tile count), every plan in
     one group (one launch per operation over the concatenated tiles, both
     captured in one hipGraph).  Baseline = plain plans (fixed threshold
@@ -208,8 +209,13 @@ def bench_batch(args, runtime, info):
     n_rhs = 16
     router = Router.load() if Router.available() else None
     shapes = synth.c5_shapes(args.count)
-    plain, learned, xs, ys_p, ys_l, samples = [], [], [], [], [], []
-    flops = bytes_plain = bytes_learned = values = 0
+    # router biases (sparse bias of the SpMV members, MFMA bias of the SpMM
+    # members, log2 ns): the group is timed at each, the faster is kept
+    biases = ((0.0, 0.0), (0.0, 1.0), (1.0, 1.0)) if router else ((0.0, 0.0),)
+    plain, xs, ys_p, ys_l, samples = [], [], [], [], []
+    cand = [[] for _ in biases]
+    cand_bytes = [0 for _ in biases]
+    flops = bytes_plain = values = 0
     t0 = time.time()
     for i, (n, parts) in enumerate(shapes):
         vbr, x, X = synth.workload_c5_matrix(i, n, parts, n_rhs=n_rhs)
@@ -219,22 +225,25 @@ def bench_batch(args, runtime, info):
             k = 1 if op == "spmv" else n_rhs
             shape = (n,) if op == "spmv" else (n, n_rhs)
             pp = runtime.Plan.from_vbr(vbr, fixed, op=op, n_rhs=k)
-            lr = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                      vbr.bpntre, k) if router else fixed)
-            # built for the group: no per-plan minimum tile count (the group
-            # fills the GPU; small matrices keep block-row-sized tiles), and
-            # smaller workgroups than the 100k tuning for these short tiles:
-            # SpMV 256 threads, SpMM N = 16 128 threads (the group measured
-            # 3.10 ms so, 3.46 with SpMV at 512, 3.65 with SpMM at 256 too)
-            pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1, wg=128 if op == "spmm" else 256)
+            for j, (sb, mb) in enumerate(biases):
+                lr = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                          vbr.bpntre, k, sparse_bias=sb if k == 1 else 0.0,
+                                          mfma_bias=mb if k > 1 else 0.0) if router else fixed)
+                # built for the group: no per-plan minimum tile count (the group
+                # fills the GPU; small matrices keep block-row-sized tiles), and
+                # smaller workgroups than the 100k tuning for these short tiles:
+                # SpMV 256 threads, SpMM N = 16 128 threads (the group measured
+                # 3.10 ms so, 3.46 with SpMV at 512, 3.65 with SpMM at 256 too)
+                pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1,
+                                           wg=128 if op == "spmm" else 256)
+                cand[j].append(pl)
+                cand_bytes[j] += pl.stats["bytes_alg"]
             plain.append(pp)
-            learned.append(pl)
             xs.append(runtime.DeviceArray.from_numpy(inp))
             ys_p.append(runtime.DeviceArray(shape))
             ys_l.append(runtime.DeviceArray(shape))
-            flops += pl.stats["flops_alg"]
+            flops += pp.stats["flops_alg"]
             bytes_plain += pp.stats["bytes_alg"]
-            bytes_learned += pl.stats["bytes_alg"]
         if i % 32 == 0:
             samples.append((i, vbr, x, X))
         del vbr
@@ -244,9 +253,19 @@ def bench_batch(args, runtime, info):
     flush = 0 if args.warm else 4 << 30
     log("c5: timing the plain plans launched one by one")
     t_plain = runtime.time_plans(plain, xs, ys_p, warmup=2, reps=args.steps, flush_bytes=flush)
-    log("c5: timing the batched group")
+    t_cands = []
+    for j, b in enumerate(biases):
+        log(f"c5: timing the batched group at router biases {b}")
+        g = runtime.Group(cand[j], xs, ys_l)
+        t_cands.append(g.time(warmup=2, reps=args.steps, flush_bytes=flush))
+        g.close()
+    kb = int(np.argmin([np.mean(t) for t in t_cands]))
+    learned, bytes_learned, t_batch = cand[kb], cand_bytes[kb], t_cands[kb]
+    for j in range(len(biases)):
+        if j != kb:
+            for p in cand[j]:
+                p.close()
     g = runtime.Group(learned, xs, ys_l)
-    t_batch = g.time(warmup=2, reps=args.steps, flush_bytes=flush)
     log("c5: timing the plain plans as a group, the learned plans one by one")
     g_plain = runtime.Group(plain, xs, ys_p)
     t_batch_plain = g_plain.time(warmup=2, reps=args.steps, flush_bytes=flush)
@@ -274,7 +293,8 @@ def bench_batch(args, runtime, info):
         "dtype": "f32 (SpMM dense blocks on bf16x6 MFMA where routed)", "data": "synthetic",
         "config": {"workload": (f"{len(shapes)} VBR matrices, n log-uniform 2000-64000, P uniform 50-800, "
                                 f"1% blocks, c2 density mix, seed 5; SpMV + SpMM N=16 each"),
-                   "routing": "learned" if router else "fixed T=15 (no router weights)",
+                   "routing": (f"learned at router biases {biases[kb]} (sparse for SpMV, MFMA for SpMM; the "
+                               f"faster group of {list(biases)})" if router else "fixed T=15 (no router weights)"),
                    "cache": "warm" if args.warm else "cold (4 GiB flush before each repetition)",
                    "stored_values": values, "plans": len(learned)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -282,6 +302,7 @@ def bench_batch(args, runtime, info):
         "cpu_baseline": None,
         "unbatched_plain_ms": ms_plain, "batched_plain_ms": float(np.mean(t_batch_plain)),
         "unbatched_learned_ms": float(np.mean(t_learned_unbatched)),
+        "batched_learned_ms_by_bias": {str(bb): float(np.mean(t)) for bb, t in zip(biases, t_cands)},
         "speedup_vs_unbatched_plain": ms_plain / ms,
         "bytes_alg_learned": bytes_learned, "bytes_alg_plain": bytes_plain, "flops_alg": flops,
         "sample_errors_vs_fp64": {"max": max(errs), "checked_products": len(errs)},
