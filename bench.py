@@ -88,7 +88,8 @@ def bench_spmm(args, runtime, info):
     FP32 vector path; MFMA = the same routing with dense blocks on the bf16
     split MFMA path (exact 3-term split, 6 passes; and hi+lo, 3 passes);
     learned = the fastest within 1e-5 of the router's plans at MFMA-route
-    biases 0, 0.5 and 1 (log2 ns), with the 3-term split."""
+    biases 0, 0.5 and 1 (log2 ns), with the 3-term split, each in 512- and
+    256-thread workgroups."""
     from sable_amd import synth
     from sable_amd.convert import threshold_routes, ROUTE_DENSE, ROUTE_DENSE_MFMA
     from oracle import c_oracle
@@ -103,9 +104,9 @@ def bench_spmm(args, runtime, info):
     Yd = runtime.DeviceArray((int(vbr.rpntr[-1]), n_rhs))
     flush = 0 if args.warm else FLUSH_BYTES
     res = {}
-    variants = [("fp32_plain", base, 3),
-                ("mfma_x6", np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base), 3),
-                ("mfma_x3", np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base), 2)]
+    variants = [("fp32_plain", base, 3, {}),
+                ("mfma_x6", np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base), 3, {}),
+                ("mfma_x3", np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base), 2, {})]
     from sable_amd.router import Router
     from sable_amd.router.router import BIAS_GRID
     if args.routing in ("learned", "both") and Router.available():
@@ -113,10 +114,14 @@ def bench_spmm(args, runtime, info):
         for sb, mb in BIAS_GRID["spmm"]:
             lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
                                      vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
-            variants.append((f"learned_x6 mfma_bias={mb:g}", lr, 3))
-    for name, routes, split in variants:
+            # 512-thread workgroups (the default) and 256: with few FP32
+            # tiles left, smaller MFMA workgroups spread the phase over more
+            # CUs (every block on MFMA: 0.161 vs 0.168 ms)
+            for wg in (512, 256):
+                variants.append((f"learned_x6 mfma_bias={mb:g} wg={wg}", lr, 3, {"wg": wg}))
+    for name, routes, split, kw in variants:
         t1 = time.time()
-        plan = runtime.Plan.from_vbr(vbr, routes.astype(np.uint8), op="spmm", n_rhs=n_rhs, mfma_split=split)
+        plan = runtime.Plan.from_vbr(vbr, routes.astype(np.uint8), op="spmm", n_rhs=n_rhs, mfma_split=split, **kw)
         insp = time.time() - t1
         times = plan.time(Xd, Yd, warmup=args.warmup, reps=args.steps, flush_bytes=flush)
         err = c_oracle.scaled_error(Yd.numpy().ravel(), Y64.ravel(), AY.ravel())
