@@ -162,12 +162,6 @@ int sable_spmm(SablePlan *plan, const float *X, float *Y, int flags);
 int sable_time(SablePlan *plan, const float *x, float *y, int warmup, int reps,
                int64_t flush_bytes, float *times_ms);
 
-/* Per-tile wall-clock durations (no reference counterpart).  Disabled in
- * this build: both return SABLE_ERR_UNSUPPORTED (router labels come from
- * event-timed launches instead). */
-int sable_plan_enable_tile_clock(SablePlan *plan, int enable);
-int sable_tile_times(SablePlan *plan, float *ns, int32_t ntiles);
-
 /* Batching (no single reference counterpart: replaces running many generated
  * programs, bench_launcher.py:85 / eval.py:30, one process each).  The
  * plans' tiles (each plan with its own x/y) run in launches over one

@@ -97,8 +97,6 @@ _SIGS = {
     "sable_spmv": (I32, [P, P, P, I32]),
     "sable_spmm": (I32, [P, P, P, I32]),
     "sable_time": (I32, [P, P, P, I32, I32, I64, F32P]),
-    "sable_plan_enable_tile_clock": (I32, [P, I32]),
-    "sable_tile_times": (I32, [P, F32P, I32]),
     "sable_group_create": (I32, [C.POINTER(P), I32, C.POINTER(P), C.POINTER(P), I32, I32, C.POINTER(P)]),
     "sable_group_launch": (I32, [P, I32]),
     "sable_group_time": (I32, [P, I32, I32, I64, F32P]),

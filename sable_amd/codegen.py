@@ -21,7 +21,14 @@ run       ``run_program(dir_name, filename)`` (or ``python -m
 
 Deviations (documented): SpMV runs in FP32 on the GPU (the reference's C is
 FP64); the SpMM RHS width is a parameter (``n_rhs``, default 512 as the
-reference); both threaded generators produce the same GPU program.
+reference); the threaded generators produce the same GPU program as the
+single-threaded ones (the thread count only shapes the host CPU program in
+the reference).  Two defects of the reference's threaded emitters are not
+reproduced: its SpMV passes ``indx[count]`` instead of
+``indx[indx_start + count]`` to two kernel shapes (src/codegen.py:615-624),
+and its SpMM gives block rows whose blocks are all sparse-routed zero work,
+so ``split_chunks`` assigns them to no thread and their rows stay zero
+(src/codegen.py:961-986, :987).
 """
 from __future__ import annotations
 
@@ -126,22 +133,70 @@ def vbr_spmv_codegen(filename: str, dir_name: str, vbr_dir: str, threads: int, b
     return time.time_ns() // 1_000_000 - t1
 
 
+# Worst-case componentwise error of one product term a*x on the MFMA route,
+# relative to |a||x|, from the bf16 terms each split keeps (u = 2^-8, the
+# unit roundoff of bf16 round to nearest): the 3-term split drops a2x3,
+# a3x2 and a3x3 (a1 + a2 + a3 == a exactly), about 2u^3; hi+lo drops a2x2
+# and the two products with the residual beyond two terms, about 3u^2.
+# FP32 accumulation adds its own rounding on top of either.
+MFMA_SPLIT_BOUND = {"bf16x6": 2.0 * 2.0 ** -24 + 2.0 ** -32, "bf16x3": 3.0 * 2.0 ** -16 + 2.0 ** -24}
+
+
+def split_chunks(values, num_chunks):
+    """src/codegen.py:165-186: greedy longest-processing-time split of the
+    indices of the nonzero entries of ``values`` into at most ``num_chunks``
+    lists (fewer when fewer entries are nonzero).  Entries are taken by value,
+    largest first (equal values keep index order), each into the first chunk
+    with the smallest running sum; zero entries are in no chunk."""
+    nonzero = [(i, v) for i, v in enumerate(values) if v != 0]
+    num_chunks = min(int(num_chunks), len(nonzero))
+    chunks = [[] for _ in range(num_chunks)]
+    sums = [0] * num_chunks
+    for i, v in sorted(nonzero, key=lambda e: e[1], reverse=True):
+        j = sums.index(min(sums))
+        chunks[j].append(i)
+        sums[j] += v
+    return chunks
+
+
+def extract_mul_nums(output) -> list:
+    """utils/utils.py:30-33: the timing fields of a program's first stdout
+    line ``name = t0,t1,...,`` (strings, empty fields dropped)."""
+    return [f for f in output.split("=")[1].split(",") if f != ""]
+
+
+def _spmm_routes(val, indx, nblocks, density, precision, tol, n_rhs):
+    if precision not in ("fp32", "bf16x6", "bf16x3"):
+        raise ValueError(f"unknown precision {precision!r}")
+    if precision != "fp32" and (n_rhs % 16 or n_rhs > 256):
+        raise ValueError(f"precision {precision!r} (the MFMA route) needs n_rhs a multiple of 16 and at most "
+                         f"256, got {n_rhs}")
+    if precision != "fp32" and MFMA_SPLIT_BOUND[precision] > tol:
+        raise ValueError(f"precision {precision!r} is bounded by {MFMA_SPLIT_BOUND[precision]:.2e} relative "
+                         f"to |A||X| per row, above the tolerance {tol:g}; use 'bf16x6' or pass a larger tol")
+    if density > 0:
+        routes = convert.threshold_routes(val, indx, density)
+    else:
+        routes = np.zeros(nblocks, dtype=np.uint8)
+    if precision != "fp32":
+        routes = np.where(routes == convert.ROUTE_DENSE, convert.ROUTE_DENSE_MFMA, routes).astype(np.uint8)
+    return routes
+
+
 def gen_single_threaded_spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, density: int, dir_name: str,
-                             filename: str, vbr_dir: str, n_rhs: int = 512, precision: str = "fp32"):
+                             filename: str, vbr_dir: str, n_rhs: int = 512, precision: str = "fp32",
+                             tol: float = 1e-5):
     """src/codegen.py:827-937 with the RHS width as a parameter.  Blocks are
     routed inline like the reference (:906-922): with density > 0 a block is
     dense iff its nonzero percentage > density, otherwise every block is
     dense.  precision: "fp32" (vector FP32 path), "bf16x6" (dense blocks on
     MFMA with the exact 3-term bf16 split, 6 passes) or "bf16x3" (hi+lo
-    split, 3 passes)."""
+    split, 3 passes).  A precision whose worst-case error bound
+    (MFMA_SPLIT_BOUND) exceeds ``tol`` is refused with ValueError, so
+    "bf16x3" needs tol >= about 4.6e-5.  Returns elapsed microseconds."""
     t1 = time.time_ns() // 1_000
     val = np.asarray(val)
-    if density > 0:
-        routes = convert.threshold_routes(val, indx, density)
-    else:
-        routes = np.zeros(len(bindx), dtype=np.uint8)
-    if precision in ("bf16x3", "bf16x6"):
-        routes = np.where(routes == convert.ROUTE_DENSE, convert.ROUTE_DENSE_MFMA, routes).astype(np.uint8)
+    routes = _spmm_routes(val, indx, len(bindx), density, precision, tol, int(n_rhs))
     arrays = dict(val=val, indx=np.asarray(indx), bindx=np.asarray(bindx), rpntr=np.asarray(rpntr),
                   cpntr=np.asarray(cpntr), bpntrb=np.asarray(bpntrb), bpntre=np.asarray(bpntre))
     meta = {"op": "spmm", "n_rhs": int(n_rhs), "bench": 1, "filename": filename,
@@ -151,13 +206,34 @@ def gen_single_threaded_spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, den
     return time.time_ns() // 1_000 - t1
 
 
+def gen_multi_threaded_spmm(threads, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, density: int,
+                            dir_name: str, filename: str, vbr_dir: str, n_rhs: int = 512,
+                            precision: str = "fp32", tol: float = 1e-5):
+    """src/codegen.py:939-1075.  The reference splits the block rows over
+    ``threads`` pthreads by split_chunks of their dense work; the GPU program
+    covers every block row in one launch whatever the thread count, so this
+    emits the single-threaded generator's program.  Block rows whose blocks
+    are all sparse-routed (zero dense work, which the reference's chunks
+    leave out, :986-987) are computed.  Returns elapsed microseconds."""
+    if int(threads) < 1:
+        raise ValueError("threads must be at least 1")
+    return gen_single_threaded_spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, density, dir_name, filename,
+                                    vbr_dir, n_rhs, precision, tol)
+
+
 def vbr_spmm_codegen(filename: str, density: int, dir_name: str, vbr_dir: str, threads: int,
-                     n_rhs: int = 512, precision: str = "fp32"):
-    """src/codegen.py:1449-1458 (reads the .vbr; returns microseconds)."""
+                     n_rhs: int = 512, precision: str = "fp32", tol: float = 1e-5):
+    """src/codegen.py:1449-1458: reads ``{vbr_dir}/{filename}.vbr`` and
+    dispatches on ``threads`` (1: single-threaded generator, else the
+    multi-threaded one).  Returns the generation time in microseconds."""
     val, indx, bindx, rpntr, cpntr, bpntrb, bpntre = read_vbr(os.path.join(vbr_dir, filename + ".vbr"))
     t1 = time.time_ns() // 1_000
-    gen_single_threaded_spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, density, dir_name, filename,
-                             vbr_dir, n_rhs, precision)
+    if threads == 1:
+        gen_single_threaded_spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, density, dir_name, filename,
+                                 vbr_dir, n_rhs, precision, tol)
+    else:
+        gen_multi_threaded_spmm(threads, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, density, dir_name,
+                                filename, vbr_dir, n_rhs, precision, tol)
     return time.time_ns() // 1_000 - t1
 
 

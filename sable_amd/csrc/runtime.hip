@@ -323,8 +323,6 @@ struct SablePlan {
     int32_t *d_queue = nullptr;
     const float **d_xptr = nullptr;
     float **d_yptr = nullptr;
-    uint64_t *d_clock = nullptr;
-    bool clock_on = false;
     // host staging for SABLE_HOST_PTRS
     float *d_xtmp = nullptr, *d_ytmp = nullptr;
     int64_t xtmp_n = 0, ytmp_n = 0;
@@ -705,7 +703,7 @@ int sable_plan_destroy(SablePlan *p)
     if (!p) return SABLE_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
     void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap,
-                    p->d_queue, p->d_xptr, p->d_yptr, p->d_clock, p->d_xtmp, p->d_ytmp, p->d_flush};
+                    p->d_queue, p->d_xptr, p->d_yptr, p->d_xtmp, p->d_ytmp, p->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
@@ -843,24 +841,6 @@ int sable_time(SablePlan *p, const float *x, float *y, int warmup, int reps, int
     if (rc) return rc;
     HIPOK(hipStreamSynchronize(p->stream));
     return SABLE_OK;
-}
-
-int sable_plan_enable_tile_clock(SablePlan *p, int enable)
-{
-    // Switched off: the first GPU run with per-tile clocks enabled went
-    // silent and was stopped, and no cause has been shown yet.  Launches
-    // never pass a clock buffer while this returns an error.
-    (void)p;
-    (void)enable;
-    return fail(SABLE_ERR_UNSUPPORTED, "per-tile clocks are disabled in this build");
-}
-
-int sable_tile_times(SablePlan *p, float *ns, int32_t ntiles)
-{
-    (void)p;
-    (void)ns;
-    (void)ntiles;
-    return fail(SABLE_ERR_UNSUPPORTED, "per-tile clocks are disabled in this build");
 }
 
 // ---- groups: many plans in persistent launches, captured in one hipGraph

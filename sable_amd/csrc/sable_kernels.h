@@ -49,7 +49,6 @@ struct SbSpmvArgs {
     const float *x0;              // single plan: x
     float *y0;                    // single plan: y
     int32_t *queue;               // [0] next tile, [1] retired workgroups
-    uint64_t *tile_clock;         // optional [2*ntiles]: start/end wall clock
     int32_t ntiles;
     int32_t mode;
 };
@@ -67,7 +66,6 @@ struct SbSpmmArgs {
     const float *x0;              // single plan: X
     float *y0;                    // single plan: Y
     int32_t *queue;
-    uint64_t *tile_clock;
     int32_t ntiles;
     int32_t mode;
     int32_t n_rhs;
@@ -75,11 +73,6 @@ struct SbSpmmArgs {
 
 // ---------------------------------------------------------------------------
 // Work distribution helpers
-
-__device__ __forceinline__ void sb_clock(uint64_t *clk, int t, int which)
-{
-    if (clk && threadIdx.x == 0) clk[2 * t + which] = wall_clock64();
-}
 
 // Runs body(t) over the tiles according to mode.  The queue counters are
 // reset by the last workgroup to retire, so the kernel can be relaunched or
@@ -304,7 +297,6 @@ template <int WG, int UNROLL, int XCAP>
 __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *xs, float *red, float *spy)
 {
     const int tid = threadIdx.x;
-    sb_clock(a.tile_clock, t, 0);
     const SbTile tile = a.tiles[t];
     const int hs = tile.nrows;
     const int tpar = tile.dense_tpar;
@@ -386,10 +378,6 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         float s = 0.f;
         for (int g = 0; g < m; ++g) s += red[g * hs + tid];
         y[tile.row0 + tid] = s + spy[tid];
-    }
-    if (a.tile_clock) {
-        __syncthreads();
-        sb_clock(a.tile_clock, t, 1);
     }
 }
 
@@ -572,7 +560,6 @@ template <int WG, int RTMAX, int NRHS>
 __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int *xrow, float4 *xs4, float *spv,
                                                  int *spx)
 {
-    sb_clock(a.tile_clock, t, 0);
     const int N = NRHS ? NRHS : a.n_rhs;
     const SbTile tile = a.tiles[t];
     const int hs = tile.nrows;
@@ -606,10 +593,6 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
                 *dst = o;
             }
         }
-    }
-    if (a.tile_clock) {
-        __syncthreads();
-        sb_clock(a.tile_clock, t, 1);
     }
 }
 

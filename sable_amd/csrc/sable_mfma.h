@@ -184,7 +184,6 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
     constexpr int NW = Cfg::NW;
     constexpr int UN = Cfg::UNITS;
-    sb_clock(a.tile_clock, t, 0);
     const int N = NRHS ? NRHS : a.n_rhs;
     const int nt = N >> 4;
     const SbTile tile = a.tiles[t];
@@ -361,10 +360,6 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
                 }
             }
         }
-    }
-    if (a.tile_clock) {
-        __syncthreads();
-        sb_clock(a.tile_clock, t, 1);
     }
 }
 

@@ -291,15 +291,6 @@ class Plan:
     def set_stream(self, stream_ptr):
         check(lib.sable_plan_set_stream(self._h, stream_ptr))
 
-    def enable_tile_clock(self, on=True):
-        check(lib.sable_plan_enable_tile_clock(self._h, int(on)))
-
-    def tile_times(self):
-        n = self.stats["ntiles"]
-        out = np.zeros(n, dtype=np.float32)
-        check(lib.sable_tile_times(self._h, out.ctypes.data_as(_lib.F32P), n))
-        return out
-
 
 def _dev_ptrs(plans, xs, ys):
     """Device pointers of every plan's x and y (DeviceArrays only: these

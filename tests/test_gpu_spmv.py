@@ -175,8 +175,8 @@ def test_device_tensors_and_timing(rt):
     assert _err(vbr, x, yd.numpy())["componentwise"] <= TOL
 
 
-@pytest.mark.skip(reason="per-tile clock path switched off: its first GPU run went silent and was "
-                         "stopped; no cause shown yet (the entry point returns SABLE_ERR_UNSUPPORTED)")
+@pytest.mark.skip(reason="per-tile clocks were removed from the library after their first GPU run went "
+                         "silent and was stopped; the cause was not shown, so the path is not run again")
 def test_tile_clock(rt):
     vbr, x, _ = synth.workload_like_c2(5000, seed=5)
     plan = rt.Plan.from_vbr(vbr, threshold_routes(vbr.val, vbr.indx, 15))
@@ -199,15 +199,6 @@ def test_planted_error_would_be_caught(rt):
     from sable_amd.vbr import VBR
     bad = VBR(val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
     assert _err(bad, x, y)["componentwise"] > TOL
-
-
-def test_tile_clock_reports_unsupported(rt):
-    from sable_amd._lib import SableError
-    vbr, x, _ = synth.workload_like_c2(3000, seed=2)
-    plan = rt.Plan.from_vbr(vbr)
-    with pytest.raises(SableError):
-        plan.enable_tile_clock(True)
-    assert _err(vbr, x, plan.spmv(x))["componentwise"] <= TOL
 
 
 def test_mixed_routes_256_threads_unroll_2(rt):
