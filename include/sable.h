@@ -90,6 +90,11 @@ typedef struct {
                                  tile_bytes is halved (down to 16 KiB) so small
                                  matrices still fill the device; 0 = 1024 for
                                  SpMV, 256 for SpMM */
+    int32_t mfma_kchunk;      /* SpMM MFMA tiles: at most about this many 32-column
+                                 k-steps per work item; a longer row tile is split
+                                 along k into chunks whose partial sums the last
+                                 chunk to finish adds in chunk order (deterministic);
+                                 0 = 32, negative = no split */
 } SablePlanOpts;
 
 typedef struct {
@@ -114,6 +119,10 @@ typedef struct {
     int64_t device_bytes;     /* device memory held by the plan */
     int32_t rtc_used;         /* 1 if the launch kernel came from hipRTC */
     int32_t mode, grid, wg;
+    int32_t ntiles_split;     /* SpMM: MFMA row tiles split along k */
+    int64_t scratch_bytes;    /* device scratch of the split tiles: partial sums and
+                                 one counter each (not counted in bytes_alg) */
+    int64_t fixup_bytes;      /* partial sums written and read back per launch */
 } SablePlanStats;
 
 /* Library. */
@@ -165,12 +174,17 @@ int sable_time(SablePlan *plan, const float *x, float *y, int warmup, int reps,
 /* Batching (no single reference counterpart: replaces running many generated
  * programs, bench_launcher.py:85 / eval.py:30, one process each).  The
  * plans' tiles (each plan with its own x/y) run in launches over one
- * LPT-ordered work list per (op, n_rhs, wg) class of members -- one launch
- * when all members share them, e.g. SpMV then SpMM for mixed groups --
- * captured once into a hipGraph and replayed by sable_group_launch.  mode
- * as in SablePlanOpts: SABLE_MODE_GRID (one workgroup per tile) measured
- * fastest; STATIC and QUEUE launch persistent workgroups.  Members built for
- * a group need no min_tiles of their own (the group fills the GPU). */
+ * LPT-ordered work list per class of members that share every option of the
+ * kernel specialisation (op, n_rhs, wg, mfma_split, unroll, use_rtc) -- one
+ * launch per phase when all members share them, e.g. SpMV then SpMM for
+ * mixed groups -- captured once into a hipGraph and replayed by
+ * sable_group_launch.  The group copies only the tile and piece descriptors:
+ * its kernels read each member's matrix from that member's own device
+ * arrays, so every member plan must stay alive (not destroyed) until the
+ * group is destroyed.  mode as in SablePlanOpts: SABLE_MODE_GRID (one
+ * workgroup per tile) measured fastest; STATIC and QUEUE launch persistent
+ * workgroups.  Members built for a group need no min_tiles of their own
+ * (the group fills the GPU). */
 int sable_group_create(SablePlan *const *plans, int nplans, const float *const *x,
                        float *const *y, int mode, int grid, SableGroup **out);
 int sable_group_launch(SableGroup *g, int sync);

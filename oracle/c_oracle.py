@@ -50,6 +50,8 @@ def load(path=None):
                                                                P, P, P, P, P, I32, P]
         lib.oracle_spmm_f32.argtypes = [I32, I32, P, P, P, P, P, P, P, P, P, I32, P]
         lib.oracle_time_spmm_f32.argtypes = [I32, I32, P, P, P, P, P, P, P, P, P, I32, P, I32, P]
+        lib.oracle_spmm_mt_f32.argtypes = [I32, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, I32]
+        lib.oracle_time_spmm_mt_f32.argtypes = [I32, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, I32, I32, P]
         lib.oracle_max_threads.restype = I32
         _libs[path] = lib
     return _libs[path]
@@ -169,9 +171,11 @@ class CProgram:
 
 
 def spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, route, X, n_rhs, lib_path=None,
-         reps=0):
+         reps=0, threads=1):
     """Reference SpMM program body (FP32).  route[k] != 0 -> sparse block.
-    With reps > 0 also returns per-run ns."""
+    With reps > 0 also returns per-run ns.  threads > 1 runs the threaded
+    program (src/codegen.py:939-1075): block rows in LPT chunks of their
+    stored size (every row in a chunk), bit-identical to one thread."""
     lib = load(lib_path)
     rp, cp = _i32(rpntr), _i32(cpntr)
     bx, bb, be = _i32(bindx), _i32(bpntrb), _i32(bpntre)
@@ -185,6 +189,19 @@ def spmm(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, route, X, n_rhs, lib_pa
     args = (len(rp) - 1, len(cp) - 1, rp.ctypes.data, cp.ctypes.data, bx.ctypes.data,
             bb.ctypes.data, be.ctypes.data, ix.ctypes.data, v.ctypes.data, r.ctypes.data,
             Xc.ctypes.data, n_rhs, Y.ctypes.data)
+    if threads > 1:
+        bb_, be_ = np.asarray(bpntrb, np.int64), np.asarray(bpntre, np.int64)
+        work = [int(ix[e] - ix[b]) if b != -1 else 0 for b, e in zip(bb_, be_)]
+        chunks = lpt_chunks([w if w else 1 for w in work], threads)
+        cptr = _i32(np.concatenate([[0], np.cumsum([len(c) for c in chunks])]))
+        rows = _i32([r for c in chunks for r in c])
+        targs = args + (cptr.ctypes.data, rows.ctypes.data, len(chunks))
+        if reps > 0:
+            t = np.zeros(reps, dtype=np.int64)
+            lib.oracle_time_spmm_mt_f32(*targs, reps, t.ctypes.data)
+            return Y, t
+        lib.oracle_spmm_mt_f32(*targs)
+        return Y
     if reps > 0:
         t = np.zeros(reps, dtype=np.int64)
         lib.oracle_time_spmm_f32(*args, reps, t.ctypes.data)

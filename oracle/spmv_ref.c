@@ -231,6 +231,67 @@ int oracle_spmm_f32(int32_t nbr, int32_t nbc, const int32_t *rpntr,
     return 0;
 }
 
+/* Threaded SpMM (src/codegen.py:939-1075 splits block rows over pthreads by
+ * split_chunks): chunk t owns the block rows rows_of[chunk_ptr[t] ..
+ * chunk_ptr[t+1]), every block row of the matrix in exactly one chunk, so
+ * each Y row is written by one thread in the single-threaded order and the
+ * result is bit-identical to oracle_spmm_f32.  Block p of row a is ordinal
+ * p (bindx is ascending within a row). */
+int oracle_spmm_mt_f32(int32_t nbr, int32_t nbc, const int32_t *rpntr,
+                       const int32_t *cpntr, const int32_t *bindx,
+                       const int32_t *bpntrb, const int32_t *bpntre,
+                       const int64_t *indx, const float *val, const uint8_t *route,
+                       const float *X, int32_t n_rhs, float *Y,
+                       const int32_t *chunk_ptr, const int32_t *rows_of, int32_t nchunks)
+{
+    const int N = n_rhs;
+    (void)nbc;
+    _Pragma("omp parallel for schedule(static, 1) num_threads(nchunks)")
+    for (int t = 0; t < nchunks; ++t) {
+        for (int32_t q = chunk_ptr[t]; q < chunk_ptr[t + 1]; ++q) {
+            const int32_t a = rows_of[q];
+            const int i0 = rpntr[a], i1 = rpntr[a + 1];
+            const int h = i1 - i0;
+            memset(Y + (size_t)i0 * N, 0, sizeof(float) * (size_t)h * (size_t)N);
+            if (bpntrb[a] == -1) continue;
+            for (int32_t p = bpntrb[a]; p < bpntre[a]; ++p) {
+                const int j0 = cpntr[bindx[p]], j1 = cpntr[bindx[p] + 1];
+                const float *v = val + indx[p];
+                const int sparse = route[p] != 0;
+                for (int i = i0; i < i1; ++i) {
+                    float *yr = Y + (size_t)i * N;
+                    for (int j = j0; j < j1; ++j) {
+                        const float a_ij = v[(j - j0) * h + (i - i0)];
+                        if (sparse && a_ij == 0.0f) continue;
+                        const float *xr = X + (size_t)j * N;
+                        for (int k = 0; k < N; ++k) yr[k] += a_ij * xr[k];
+                    }
+                }
+            }
+        }
+    }
+    (void)nbr;
+    return 0;
+}
+
+int oracle_time_spmm_mt_f32(int32_t nbr, int32_t nbc, const int32_t *rpntr,
+                            const int32_t *cpntr, const int32_t *bindx,
+                            const int32_t *bpntrb, const int32_t *bpntre,
+                            const int64_t *indx, const float *val, const uint8_t *route,
+                            const float *X, int32_t n_rhs, float *Y,
+                            const int32_t *chunk_ptr, const int32_t *rows_of, int32_t nchunks,
+                            int32_t reps, int64_t *times_ns)
+{
+    for (int it = 0; it < reps; ++it) {
+        int64_t t1 = now_ns();
+        oracle_spmm_mt_f32(nbr, nbc, rpntr, cpntr, bindx, bpntrb, bpntre, indx, val, route, X, n_rhs, Y,
+                           chunk_ptr, rows_of, nchunks);
+        int64_t t2 = now_ns();
+        times_ns[it] = t2 - t1;
+    }
+    return 0;
+}
+
 int oracle_time_spmm_f32(int32_t nbr, int32_t nbc, const int32_t *rpntr,
                          const int32_t *cpntr, const int32_t *bindx,
                          const int32_t *bpntrb, const int32_t *bpntre,

@@ -268,8 +268,10 @@ def run_program(dir_name, filename, x=None):
     return times, yd.numpy()
 
 
-def print_program_output(filename, times, y, op="spmv", out=sys.stdout):
-    """The generated binary's stdout (src/codegen.py:440-447, :927-934)."""
+def print_program_output(filename, times, y, op="spmv", out=None):
+    """The generated binary's stdout (src/codegen.py:440-447, :927-934),
+    written to ``out`` (default: the current sys.stdout)."""
+    out = sys.stdout if out is None else out
     if op == "spmv":
         out.write(f"{filename} = " + "".join(f"{t}," for t in times) + "\n")
         for v in np.asarray(y, dtype=np.float64).ravel():

@@ -45,6 +45,7 @@ SablePlanOpts normalize_opts(const SablePlanOpts *in)
     // SpMV tiles are light (stream-bound); SpMM tiles carry MFMA work and
     // gain from tall tiles (each staged X k-step serves every row)
     if (o.min_tiles <= 0) o.min_tiles = (o.op == SABLE_OP_SPMV) ? 1024 : 256;
+    if (o.mfma_kchunk == 0) o.mfma_kchunk = 32;
     return o;
 }
 
@@ -426,7 +427,20 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
         const int vcap = spmm ? std::max(1, std::min(o.xcap, SB_SPMM_XS_FLOATS / o.n_rhs)) : o.xcap;
         auto dgroups = group_columns(dense, v, vcap);
         auto sgroups = group_columns(sparse, v, vcap);
-        auto mgroups = group_columns(mfma, v, o.xcap);
+        // MFMA columns: with k chunks, groups of at most kper 32-column
+        // k-steps, kper = ceil(K / ceil(K / kchunk)) for the block row's K
+        // k-steps, so its chunks are even; each group is then one work item
+        int mcap = o.xcap;
+        const bool ksplit = spmm && !mfma.empty() && o.mfma_kchunk > 0;
+        if (ksplit) {
+            int64_t W = 0;
+            for (int64_t k : mfma) W += v.cpntr[v.bindx[k] + 1] - v.cpntr[v.bindx[k]];
+            const int64_t K = (W + 31) / 32;
+            const int64_t nc = std::max<int64_t>(1, (K + o.mfma_kchunk - 1) / o.mfma_kchunk);
+            const int64_t kper = (K + nc - 1) / nc;
+            mcap = (int)std::min<int64_t>((o.xcap / 32) * 32, 32 * kper);
+        }
+        auto mgroups = group_columns(mfma, v, mcap);
 
         // sparse nonzeros per row of the block row (for costing / lanes)
         std::vector<int32_t> row_nnz(h, 0);
@@ -448,12 +462,16 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
         // sparse; MFMA).  y_accum: the tile adds to Y rows an earlier MFMA
         // tile of the same rows wrote.
         const std::vector<std::vector<ColSeg>> none;
-        auto emit = [&](int32_t s0, int32_t hs, bool use_vec, bool use_mfma, int32_t y_accum) {
+        auto emit = [&](int32_t s0, int32_t hs, bool use_vec, bool use_mfma, int32_t y_accum, int mg_only = -1) {
             SbTile t{};
             t.row0 = r0 + s0;
             t.nrows = hs;
             t.mat = 0;
             t.y_accum = y_accum;
+            t.ksplit = 1;
+            t.kpart = 0;
+            t.fix = -1;
+            t.part_blk = 0;
             // sparse rows of the segment, per sparse column group
             std::vector<SparseRows> sp;
             for (size_t gi = 0; use_vec && gi < sgroups.size(); ++gi) {
@@ -529,7 +547,9 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                 cost += 4.0 * hs * pc.ncols;
             }
             // MFMA pieces (SpMM): FP32 in A-operand register order
-            for (auto &g : (use_mfma ? mgroups : none)) {
+            for (size_t gi = 0; use_mfma && gi < mgroups.size(); ++gi) {
+                if (mg_only >= 0 && (int)gi != mg_only) continue;
+                const auto &g = mgroups[gi];
                 SbPiece pc{};
                 pc.kind = SB_PIECE_MFMA;
                 pc.ncols = group_width(g);
@@ -626,8 +646,34 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             int64_t nseg = (h + mfma_rows - 1) / mfma_rows;
             const int64_t want = (int64_t)std::ceil(4.0 * h * (double)wm / (double)o.tile_bytes);
             const auto st = cut(std::max(nseg, want), true);
-            for (size_t s = 0; s + 1 < st.size(); ++s)
-                if (st[s + 1] > st[s]) emit(st[s], st[s + 1] - st[s], false, true, 0);
+            const int32_t nc = (int32_t)mgroups.size();
+            for (size_t s = 0; s + 1 < st.size(); ++s) {
+                if (st[s + 1] <= st[s]) continue;
+                const int32_t hs = st[s + 1] - st[s];
+                if (!ksplit || nc == 1) {
+                    emit(st[s], hs, false, true, 0);
+                    continue;
+                }
+                // one work item per k chunk; scratch for nc partials of
+                // (16-row units x N/16 column tiles) 1 KiB blocks, one counter
+                const int64_t blocks = (int64_t)((hs + 15) / 16) * (P.n_rhs / 16);
+                if (P.part_blocks + nc * blocks > INT32_MAX) {
+                    err = "partial-sum scratch exceeds 2^31 KiB";
+                    return SABLE_ERR_UNSUPPORTED;
+                }
+                for (int32_t c = 0; c < nc; ++c) {
+                    emit(st[s], hs, false, true, 0, c);
+                    SbTile &tt = P.tiles.back();
+                    tt.ksplit = nc;
+                    tt.kpart = c;
+                    tt.fix = P.nfix;
+                    tt.part_blk = (int32_t)P.part_blocks;
+                }
+                P.part_blocks += nc * blocks;
+                P.nfix += 1;
+                S.ntiles_split += 1;
+                S.fixup_bytes += (2 * nc) * blocks * 1024;
+            }
             if (!dgroups.empty() || !sgroups.empty()) {
                 const auto sv = vec_segments(4.0 * h * (double)wd + 6.0 * sp_total + 4.0 * (h + 1) * sgroups.size());
                 for (size_t s = 0; s + 1 < sv.size(); ++s)
@@ -672,6 +718,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     S.bytes_alg = 4 * S.dense_values + 4 * S.mfma_values + 6 * S.sparse_nnz + 4 * S.rowptr_entries +
                   S.descriptor_bytes + 4LL * P.ncols * P.n_rhs + 4LL * P.nrows * P.n_rhs;
     S.flops_alg = 2 * S.nnz * (int64_t)P.n_rhs;
+    S.scratch_bytes = P.part_blocks * 1024 + 4LL * P.nfix;
     S.wg = o.wg;
     S.mode = o.mode;
     return SABLE_OK;

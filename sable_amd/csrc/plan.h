@@ -25,6 +25,8 @@ struct HostPlan {
     std::vector<int32_t> xmap;    // SpMV: x column per staged slot (-1 = zero pad)
     std::vector<double> tile_cost;   // estimated bytes per tile (ordering)
     std::vector<int32_t> tile_phase; // 0: SpMM MFMA tile (first launch), 1: the rest
+    int64_t part_blocks = 0;         // 1 KiB partial-sum blocks of the split MFMA tiles
+    int32_t nfix = 0;                // counters of the split MFMA tiles
     SablePlanStats stats{};
 };
 

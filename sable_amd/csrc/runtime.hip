@@ -321,8 +321,10 @@ struct SablePlan {
     int32_t *d_srowptr = nullptr;
     int32_t *d_xmap = nullptr;
     int32_t *d_queue = nullptr;
-    const float **d_xptr = nullptr;
-    float **d_yptr = nullptr;
+    // scratch of the MFMA tiles split along k: partial sums, counters (zero
+    // between launches: the last chunk of a tile resets its counter)
+    float *d_part = nullptr;
+    int32_t *d_fix = nullptr;
     // host staging for SABLE_HOST_PTRS
     float *d_xtmp = nullptr, *d_ytmp = nullptr;
     int64_t xtmp_n = 0, ytmp_n = 0;
@@ -334,21 +336,18 @@ struct SablePlan {
     int64_t device_bytes = 0;
 };
 
-// One persistent launch of a group: the members that share op, RHS width and
-// workgroup size, their descriptors and data concatenated on the device.
+// One launch (per phase) of a group: the members that share every option
+// the kernel specialisation depends on (op, RHS width, workgroup size, MFMA
+// split, SpMV unroll, hipRTC use).  Their tile and piece descriptors are
+// concatenated on the device; their data stays in the members' own arrays,
+// reached through a per-matrix buffer table, so a group copies no matrix.
 struct GroupPart {
     int32_t op = SABLE_OP_SPMV, n_rhs = 1, wg = 256, mode = SABLE_MODE_QUEUE, grid = 0, ntiles = 0;
     SbTile *d_tiles = nullptr;
     SbPiece *d_pieces = nullptr;
     SbXRange *d_xr = nullptr;
-    float *d_dval = nullptr;
-    float *d_sval = nullptr;
-    uint16_t *d_sidx = nullptr;
-    int32_t *d_srowptr = nullptr;
-    int32_t *d_xmap = nullptr;
+    SbMatBufs *d_mats = nullptr;
     int32_t *d_queue = nullptr;
-    const float **d_xptr = nullptr;
-    float **d_yptr = nullptr;
     std::vector<Phase> phases;
 };
 
@@ -509,7 +508,7 @@ int launch_raw(bool rtc, hipFunction_t fn, const void *aot, int grid, int wg, vo
 int launch_phase(int op, int n_rhs, const Phase &ph, int wg, int mode, const SbTile *tiles,
                  const SbPiece *pieces, const SbXRange *xr, const float *dval, const float *sval,
                  const uint16_t *sidx, const int32_t *srowptr, const int32_t *xmap, int32_t *queue,
-                 const float *const *xv, float *const *yv, const float *x, float *y, hipStream_t st,
+                 const SbMatBufs *mats, const float *x, float *y, float *part, int32_t *fix, hipStream_t st,
                  hipEvent_t e0, hipEvent_t e1)
 {
     const int n = ph.t1 - ph.t0;
@@ -519,14 +518,14 @@ int launch_phase(int op, int n_rhs, const Phase &ph, int wg, int mode, const SbT
         SbSpmvArgs a{};
         a.tiles = tiles + ph.t0; a.pieces = pieces; a.xr = xr;
         a.dval = dval; a.sval = sval; a.sidx = sidx; a.srowptr = srowptr; a.xmap = xmap;
-        a.xvec = xv; a.yvec = yv; a.x0 = x; a.y0 = y; a.queue = queue;
+        a.mats = mats; a.x0 = x; a.y0 = y; a.queue = queue;
         a.ntiles = n; a.mode = mode;
         return launch_raw(ph.rtc, ph.fn, ph.aot, grid, wg, &a, st, e0, e1);
     }
     SbSpmmArgs a{};
     a.tiles = tiles + ph.t0; a.pieces = pieces; a.xr = xr;
     a.dval = dval; a.sval = sval; a.sidx = sidx; a.srowptr = srowptr;
-    a.xmat = xv; a.ymat = yv; a.x0 = x; a.y0 = y; a.queue = queue;
+    a.mats = mats; a.x0 = x; a.y0 = y; a.queue = queue; a.part = part; a.fix = fix;
     a.ntiles = n; a.mode = mode; a.n_rhs = n_rhs;
     return launch_raw(ph.rtc, ph.fn, ph.aot, grid, wg, &a, st, e0, e1);
 }
@@ -543,7 +542,7 @@ int plan_launch(SablePlan *p, const float *x, float *y, hipEvent_t e0 = nullptr,
         const Phase &ph = p->phases[i];
         CHECKRC(launch_phase(p->host.op, p->host.n_rhs, ph, p->opts.wg, p->opts.mode, p->d_tiles, p->d_pieces,
                              p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap, p->d_queue, nullptr,
-                             nullptr, x, y, p->stream, attached ? e0 : nullptr, attached ? e1 : nullptr));
+                             x, y, p->d_part, p->d_fix, p->stream, attached ? e0 : nullptr, attached ? e1 : nullptr));
     }
     if (e1 && !attached) HIPOK(hipEventRecord(e1, p->stream));
     return SABLE_OK;
@@ -677,6 +676,12 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     CHECKRC(upload(&p->d_xmap, h.xmap, bytes));
     HIPOK(hipMalloc((void **)&p->d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(p->d_queue, 0, 2 * sizeof(int32_t)));
+    if (h.nfix > 0) {
+        HIPOK(hipMalloc((void **)&p->d_part, (size_t)h.part_blocks * 1024));
+        HIPOK(hipMalloc((void **)&p->d_fix, (size_t)h.nfix * sizeof(int32_t)));
+        HIPOK(hipMemset(p->d_fix, 0, (size_t)h.nfix * sizeof(int32_t)));
+        bytes += h.part_blocks * 1024 + 4LL * h.nfix;
+    }
     HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     p->own_stream = true;
     p->stats.rtc_used = 1;
@@ -703,7 +708,7 @@ int sable_plan_destroy(SablePlan *p)
     if (!p) return SABLE_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
     void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap,
-                    p->d_queue, p->d_xptr, p->d_yptr, p->d_xtmp, p->d_ytmp, p->d_flush};
+                    p->d_queue, p->d_part, p->d_fix, p->d_xtmp, p->d_ytmp, p->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
@@ -849,16 +854,16 @@ namespace {
 
 void free_part(GroupPart &q)
 {
-    void *bufs[] = {q.d_tiles, q.d_pieces, q.d_xr, q.d_dval, q.d_sval, q.d_sidx, q.d_srowptr,
-                    q.d_xmap, q.d_queue, q.d_xptr, q.d_yptr};
+    void *bufs[] = {q.d_tiles, q.d_pieces, q.d_xr, q.d_mats, q.d_queue};
     for (void *b : bufs)
         if (b) hipFree(b);
     q = GroupPart();
 }
 
-// Builds one part from members idx (all sharing op, n_rhs and wg): their
-// descriptors rebased and concatenated in one global LPT order, their data
-// copied back to back on the device, one kernel specialisation.
+// Builds one part from members idx (all sharing the kernel options): their
+// tile and piece descriptors concatenated in one global LPT order, with the
+// tiles' matrix slots pointing into a table of the members' own device
+// arrays and x/y, one kernel specialisation.
 int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float *const *x, float *const *y,
                int mode, int grid, GroupPart &q)
 {
@@ -872,46 +877,44 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     std::vector<SbXRange> xr;
     std::vector<double> cost;
     std::vector<int32_t> phase;
-    int64_t dval_n = 0, sval_n = 0, srow_n = 0, xmap_n = 0;
+    std::vector<SbMatBufs> mats;
     for (size_t m = 0; m < idx.size(); ++m) {
         SablePlan *p = plans[idx[m]];
+        if (pieces.size() + p->host.pieces.size() > (size_t)INT32_MAX ||
+            xr.size() + p->host.xr.size() > (size_t)INT32_MAX)
+            return fail(SABLE_ERR_UNSUPPORTED, "group descriptor arrays exceed 2^31 entries");
         const int32_t pbase = (int32_t)pieces.size(), xbase = (int32_t)xr.size();
         for (auto pc : p->host.pieces) {
             pc.xr_begin += xbase;
             pc.xr_end += xbase;
-            if (pc.kind == SB_PIECE_DENSE || pc.kind == SB_PIECE_MFMA) {
-                pc.data_off += dval_n;
-            } else {
-                pc.data_off += sval_n;   // multiples of 256 entries keep SpMV float4 alignment
-                pc.rowptr_off += (int32_t)srow_n;
-            }
-            pc.xmap_off += (int32_t)xmap_n;
             if (pc.stage_len > 0) pc.grp_end += pbase;   // first piece of a staging group
-            pieces.push_back(pc);
+            pieces.push_back(pc);                        // data, row pointer and map offsets stay the member's
         }
         xr.insert(xr.end(), p->host.xr.begin(), p->host.xr.end());
         for (size_t t = 0; t < p->host.tiles.size(); ++t) {
             SbTile tt = p->host.tiles[t];
             tt.piece_begin += pbase;
             tt.piece_end += pbase;
-            if (tt.stage_len > 0) tt.stage_map += (int32_t)xmap_n;
             tt.mat = (int32_t)m;
-            if (tt.sp_map >= 0) tt.sp_map += (int32_t)srow_n;
             tiles.push_back(tt);
             cost.push_back(p->host.tile_cost[t]);
             phase.push_back((int32_t)t < p->stats.ntiles_mfma ? 0 : 1);
         }
-        dval_n += p->stats.dense_padded + p->stats.mfma_padded;
-        sval_n += p->stats.sparse_padded;
-        srow_n += p->stats.rowptr_entries;
-        xmap_n += p->stats.xmap_entries;
+        SbMatBufs b{};
+        b.dval = p->d_dval;
+        b.sval = p->d_sval;
+        b.sidx = p->d_sidx;
+        b.srowptr = p->d_srowptr;
+        b.xmap = p->d_xmap;
+        b.x = x[idx[m]];
+        b.y = y[idx[m]];
+        b.part = p->d_part;
+        b.fix = p->d_fix;
+        mats.push_back(b);
     }
-    if (xmap_n > INT32_MAX || srow_n > INT32_MAX || pieces.size() > (size_t)INT32_MAX)
-        return fail(SABLE_ERR_UNSUPPORTED, "group index arrays exceed 2^31 entries");
-    // global LPT order
+    // global LPT order: MFMA tiles first (first launch), largest first in each phase
     std::vector<int32_t> order(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
-    // MFMA tiles first (first launch), largest first in each phase
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         if (phase[a] != phase[b]) return phase[a] < phase[b];
         return cost[a] > cost[b];
@@ -925,39 +928,9 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     CHECKRC(upload(&q.d_tiles, sorted, bytes));
     CHECKRC(upload(&q.d_pieces, pieces, bytes));
     CHECKRC(upload(&q.d_xr, xr, bytes));
-    auto cat = [&](auto **dst, int64_t n, auto getter, size_t esz) -> int {
-        *dst = nullptr;
-        if (n == 0) return SABLE_OK;
-        HIPOK(hipMalloc((void **)dst, n * esz));
-        int64_t off = 0;
-        for (int i : idx) {
-            auto pr = getter(plans[i]);
-            if (pr.second) {
-                HIPOK(hipMemcpy((char *)*dst + off * esz, pr.first, pr.second * esz, hipMemcpyDeviceToDevice));
-                off += pr.second;
-            }
-        }
-        return SABLE_OK;
-    };
-    using P2 = std::pair<const void *, int64_t>;
-    CHECKRC(cat(&q.d_dval, dval_n,
-                [](SablePlan *p) { return P2(p->d_dval, p->stats.dense_padded + p->stats.mfma_padded); }, 4));
-    CHECKRC(cat(&q.d_sval, sval_n, [](SablePlan *p) { return P2(p->d_sval, p->stats.sparse_padded); }, 4));
-    CHECKRC(cat(&q.d_sidx, sval_n, [](SablePlan *p) { return P2(p->d_sidx, p->stats.sparse_padded); }, 2));
-    CHECKRC(cat(&q.d_srowptr, srow_n, [](SablePlan *p) { return P2(p->d_srowptr, p->stats.rowptr_entries); }, 4));
-    CHECKRC(cat(&q.d_xmap, xmap_n, [](SablePlan *p) { return P2(p->d_xmap, p->stats.xmap_entries); }, 4));
+    CHECKRC(upload(&q.d_mats, mats, bytes));
     HIPOK(hipMalloc((void **)&q.d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(q.d_queue, 0, 2 * sizeof(int32_t)));
-    std::vector<const float *> xs;
-    std::vector<float *> ys;
-    for (int i : idx) {
-        xs.push_back(x[i]);
-        ys.push_back(y[i]);
-    }
-    HIPOK(hipMalloc((void **)&q.d_xptr, xs.size() * sizeof(float *)));
-    HIPOK(hipMalloc((void **)&q.d_yptr, ys.size() * sizeof(float *)));
-    HIPOK(hipMemcpy(q.d_xptr, xs.data(), xs.size() * sizeof(float *), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(q.d_yptr, ys.data(), ys.size() * sizeof(float *), hipMemcpyHostToDevice));
     // kernel: the specialisation machinery over a host plan view of the part
     sable::HostPlan view;
     view.op = q.op;
@@ -973,8 +946,8 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
 int launch_part(const GroupPart &q, hipStream_t st)
 {
     for (const Phase &ph : q.phases)
-        CHECKRC(launch_phase(q.op, q.n_rhs, ph, q.wg, q.mode, q.d_tiles, q.d_pieces, q.d_xr, q.d_dval, q.d_sval, q.d_sidx,
-                             q.d_srowptr, q.d_xmap, q.d_queue, q.d_xptr, q.d_yptr, nullptr, nullptr, st, nullptr,
+        CHECKRC(launch_phase(q.op, q.n_rhs, ph, q.wg, q.mode, q.d_tiles, q.d_pieces, q.d_xr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, q.d_queue, q.d_mats, nullptr, nullptr, nullptr, nullptr, st, nullptr,
                              nullptr));
     return SABLE_OK;
 }
@@ -989,12 +962,23 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
     for (int i = 0; i < nplans; ++i)
         if (!plans[i] || plans[i]->host_only) return fail(SABLE_ERR_INVALID, "group member must be a device plan");
+    // a plan's split MFMA tiles own one set of partial sums and counters, so
+    // a plan listed twice would have two launches share them
+    for (int i = 0; i < nplans; ++i)
+        for (int j = 0; j < i; ++j)
+            if (plans[i] == plans[j] && plans[i]->d_fix)
+                return fail(SABLE_ERR_INVALID, "a plan with split MFMA tiles is listed twice in the group");
     std::unique_ptr<SableGroup> g(new SableGroup());
-    // members sharing (op, n_rhs, wg) form one part, in order of first appearance
+    // members sharing every option of the kernel specialisation form one
+    // part, in order of first appearance (a part runs its first member's
+    // specialisation, so a member never runs another split or unroll)
     std::vector<std::vector<int>> members;
-    std::vector<std::array<int, 3>> keys;
+    std::vector<std::array<int, 6>> keys;
     for (int i = 0; i < nplans; ++i) {
-        const std::array<int, 3> k = {plans[i]->host.op, plans[i]->host.n_rhs, plans[i]->opts.wg};
+        const SablePlanOpts &o = plans[i]->opts;
+        const bool spmm = plans[i]->host.op == SABLE_OP_SPMM;
+        const std::array<int, 6> k = {plans[i]->host.op, plans[i]->host.n_rhs, o.wg, spmm ? o.mfma_split : 0,
+                                      spmm ? 0 : o.unroll, o.use_rtc};
         size_t j = 0;
         while (j < keys.size() && keys[j] != k) ++j;
         if (j == keys.size()) {

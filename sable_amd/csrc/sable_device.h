@@ -37,6 +37,15 @@ struct SbTile {
     int32_t y_accum;         // SpMM: 1 = add to Y rows an MFMA tile wrote before
     int32_t stage_map;       // SpMV: xmap_off and stage_len of the first staging group
     int32_t stage_len;       // (copies, so its map loads follow the tile read directly)
+    // SpMM MFMA row tiles split along k: this work item is chunk kpart of
+    // ksplit (1 = not split).  Every chunk writes its partial sums to scratch
+    // block part_blk + kpart * (16-row units * N/16) (1 KiB blocks, in
+    // accumulator order) and counts itself in counter fix; the last to
+    // arrive adds the chunks in chunk order and writes the rows.
+    int32_t ksplit;
+    int32_t kpart;
+    int32_t fix;
+    int32_t part_blk;
 };
 
 // SpMV sparse slices: steps per pipelined batch; and the batches the SpMV

@@ -35,6 +35,21 @@ typedef unsigned int sb_u2 __attribute__((ext_vector_type(2)));
 #define SB_MODE_STATIC 1    // persistent, tile = wg + k * grid
 #define SB_MODE_QUEUE 2     // persistent, atomic work queue (self-resetting)
 
+// The device arrays of one matrix of a group: each member keeps its own
+// arrays (the group copies none of them) and a tile finds them through its
+// matrix slot.  Piece and tile offsets are relative to these arrays.
+struct SbMatBufs {
+    const float *dval;
+    const float *sval;
+    const uint16_t *sidx;
+    const int32_t *srowptr;
+    const int32_t *xmap;          // SpMV only
+    const float *x;               // x (SpMV) or X [ncols x n_rhs] (SpMM)
+    float *y;                     // y (SpMV) or Y [nrows x n_rhs] (SpMM)
+    float *part;                  // SpMM: partial sums of the split MFMA tiles
+    int32_t *fix;                 // SpMM: one arrival counter per split MFMA tile (zero between launches)
+};
+
 struct SbSpmvArgs {
     const SbTile *tiles;
     const SbPiece *pieces;
@@ -44,10 +59,11 @@ struct SbSpmvArgs {
     const uint16_t *sidx;
     const int32_t *srowptr;
     const int32_t *xmap;          // x column per staged slot (-1: zero pad)
-    const float *const *xvec;     // per matrix slot (batched plans) or null
-    float *const *yvec;           // per matrix slot (batched plans) or null
+    const SbMatBufs *mats;        // per matrix slot (groups), or null: the fields above
     const float *x0;              // single plan: x
     float *y0;                    // single plan: y
+    float *part;                  // unused (SpMM only)
+    int32_t *fix;
     int32_t *queue;               // [0] next tile, [1] retired workgroups
     int32_t ntiles;
     int32_t mode;
@@ -61,15 +77,34 @@ struct SbSpmmArgs {
     const float *sval;
     const uint16_t *sidx;
     const int32_t *srowptr;
-    const float *const *xmat;     // per matrix slot: X [ncols x n_rhs] row-major, or null
-    float *const *ymat;           // per matrix slot: Y [nrows x n_rhs] row-major, or null
+    const SbMatBufs *mats;        // per matrix slot (groups), or null: the fields above
     const float *x0;              // single plan: X
     float *y0;                    // single plan: Y
+    float *part;                  // single plan: partial sums of the split MFMA tiles
+    int32_t *fix;                 // single plan: their arrival counters
     int32_t *queue;
     int32_t ntiles;
     int32_t mode;
     int32_t n_rhs;
 };
+
+// The arrays a tile reads: its matrix slot's in a group, else the plan's.
+template <typename Args>
+__device__ __forceinline__ SbMatBufs sb_mat_bufs(const Args &a, int mat, const int32_t *xmap)
+{
+    if (a.mats) return a.mats[mat];
+    SbMatBufs b;
+    b.dval = a.dval;
+    b.sval = a.sval;
+    b.sidx = a.sidx;
+    b.srowptr = a.srowptr;
+    b.xmap = xmap;
+    b.x = a.x0;
+    b.y = a.y0;
+    b.part = a.part;
+    b.fix = a.fix;
+    return b;
+}
 
 // ---------------------------------------------------------------------------
 // Work distribution helpers
@@ -301,8 +336,9 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
     const int hs = tile.nrows;
     const int tpar = tile.dense_tpar;
     const int M = tile.dense_mcols;
-    const float *__restrict__ x = a.xvec ? a.xvec[tile.mat] : a.x0;
-    float *__restrict__ y = a.yvec ? a.yvec[tile.mat] : a.y0;
+    const SbMatBufs mb = sb_mat_bufs(a, tile.mat, a.xmap);
+    const float *__restrict__ x = mb.x;
+    float *__restrict__ y = mb.y;
 
     SbStreamLane L;
     {
@@ -333,7 +369,7 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         head = false;
         if (slen > XCAP + 4) break;            // never built (the host and kernel choice check)
         __syncthreads();                       // previous group done with xs
-        sb_stage_map<WG>(a.xmap + smap, slen, x, xs);
+        sb_stage_map<WG>(mb.xmap + smap, slen, x, xs);
         __syncthreads();
         const int pe = first.grp_end;
         if (pe <= pb) break;                   // never built
@@ -341,13 +377,14 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         for (int p = pb; p < pe; ++p) {
             const SbPiece pc = a.pieces[p];
             if (pc.kind == SB_PIECE_DENSE) {
-                sb_dense_piece<UNROLL>(reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off),
+                sb_dense_piece<UNROLL>(reinterpret_cast<const sb_f4 *>(mb.dval + pc.data_off),
                                        (hs * pc.ncols + 3) >> 2, tpar, M, pc.ncols, xs + base, L, a0, a1, a2, a3);
             } else if (sw >= 0) {
                 // this wavefront's slices sw, sw + nsw, ...
                 for (int sl = sw; sl < nsl; sl += nsw) {
-                    sb_sparse_slice<SB_SPUNROLL>(a.sval + pc.data_off, a.sidx + pc.data_off, a.srowptr + pc.rowptr_off,
-                                                 a.srowptr + tile.sp_map, hs, tile.sp_glog, sl, pc.ncols, xs + base,
+                    sb_sparse_slice<SB_SPUNROLL>(mb.sval + pc.data_off, mb.sidx + pc.data_off,
+                                                 mb.srowptr + pc.rowptr_off, mb.srowptr + tile.sp_map, hs,
+                                                 tile.sp_glog, sl, pc.ncols, xs + base,
                                                  spy);
                 }
             }
@@ -478,7 +515,7 @@ __device__ __forceinline__ void sb_fma4(float4 &acc, float a, const float4 &x)
 // then each thread walks its rows' entries.  spv/spx are LDS scratch of
 // SB_SPMM_SPCHUNK entries.
 template <int WG, int RTMAX>
-__device__ __forceinline__ void sb_spmm_vec_piece(const SbSpmmArgs &a, const SbPiece &pc, int hs, int N,
+__device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPiece &pc, int hs, int N,
                                                   const SbVecMap &m, const float4 *xs4, float4 *acc, float *spv,
                                                   int *spx)
 {
@@ -564,8 +601,9 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
     const SbTile tile = a.tiles[t];
     const int hs = tile.nrows;
     const SbVecMap m = sb_vec_map(hs, N, WG);
-    const float *__restrict__ X = a.xmat ? a.xmat[tile.mat] : a.x0;
-    float *__restrict__ Y = a.ymat ? a.ymat[tile.mat] : a.y0;
+    const SbMatBufs mb = sb_mat_bufs(a, tile.mat, nullptr);
+    const float *__restrict__ X = mb.x;
+    float *__restrict__ Y = mb.y;
     float4 acc[RTMAX];
 #pragma unroll
     for (int i = 0; i < RTMAX; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -577,7 +615,7 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
         __syncthreads();
         sb_stage_xrows<WG>(X, N, xrow, pc.ncols, xs4);
         __syncthreads();
-        sb_spmm_vec_piece<WG, RTMAX>(a, pc, hs, N, m, xs4, acc, spv, spx);
+        sb_spmm_vec_piece<WG, RTMAX>(mb, pc, hs, N, m, xs4, acc, spv, spx);
     }
     if (m.active) {
 #pragma unroll

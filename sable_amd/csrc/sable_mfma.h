@@ -166,6 +166,44 @@ struct SbMfmaCfg {
     static constexpr int SMEM_BYTES = 2 * BUF_STRIDE * 2;          // double-buffered
 };
 
+// The end of a chunk of a split MFMA tile (the chunks of one row tile are
+// separate work items over consecutive k ranges).  Every chunk has stored
+// its partial sums; it makes them visible device-wide (release fence), then
+// counts itself in the tile's counter.  The chunk that arrives last (any
+// chunk: the order of arrival does not matter) makes the others' stores
+// visible to itself (acquire fence), adds the partials in chunk order
+// 0, 1, ..., ksplit-1 -- so Y is the same whichever chunk finishes last --,
+// writes the rows and resets the counter for the next launch.  No wave waits
+// on another workgroup.
+template <int NW>
+__device__ __forceinline__ void sb_mfma_fixup(const SbTile &tile, const SbMatBufs &mb, float *Y, int N, int nt,
+                                              int mt, int *fixflag)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) *fixflag = atomicAdd(&mb.fix[tile.fix], 1) == tile.ksplit - 1;
+    __syncthreads();
+    const bool last = *fixflag != 0;
+    __syncthreads();                               // fixflag is read before the next tile reuses it
+    if (!last) return;
+    __threadfence();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int nblk = mt * nt;
+    const sb_f32x4 *__restrict__ part = reinterpret_cast<const sb_f32x4 *>(mb.part) + (int64_t)tile.part_blk * 64;
+    for (int b = wave; b < nblk; b += NW) {
+        sb_f32x4 s = part[(int64_t)b * 64 + lane];
+        for (int c = 1; c < tile.ksplit; ++c) s += part[((int64_t)c * nblk + b) * 64 + lane];
+        const int m = b / nt, j = b - m * nt;
+        const int rbase = m * 16 + 4 * (lane >> 4);
+        const int col = j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (rbase + r < tile.nrows) Y[(size_t)(tile.row0 + rbase + r) * N + col] = s[r];
+    }
+    if (threadIdx.x == 0) atomicExch(&mb.fix[tile.fix], 0);
+}
+
 // One MFMA tile (only MFMA pieces; it writes its Y rows).  Work units are
 // (16-row tile mi, column group gi) pairs: ng column groups of nt/ng
 // 16-column tiles, ng the smallest power of two dividing nt that gives every
@@ -179,7 +217,8 @@ struct SbMfmaCfg {
 // were issued a k-step earlier, and one barrier per k-step orders the
 // buffers.
 template <int WG, int NRHS, int SPLIT>
-__device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem)
+__device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem,
+                                                  int *fixflag)
 {
     using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
     constexpr int NW = Cfg::NW;
@@ -201,10 +240,12 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     const int units = mt * ng;
     const int mine = units > wave ? (units - wave + NW - 1) / NW : 0;   // units of this wavefront
     const int passes = (mine + UN - 1) / UN;
-    const float *__restrict__ X = a.xmat ? a.xmat[tile.mat] : a.x0;
-    float *__restrict__ Y = a.ymat ? a.ymat[tile.mat] : a.y0;
+    const SbMatBufs mb = sb_mat_bufs(a, tile.mat, nullptr);
+    const float *__restrict__ X = mb.x;
+    float *__restrict__ Y = mb.y;
     uint16_t *B = reinterpret_cast<uint16_t *>(smem);
     const int nb = lane & 15, kb = (lane >> 4) * 8;
+    const bool split = tile.ksplit > 1;
 
     // every wavefront runs the same number of passes (the staging barriers)
     int all_passes = (((units + NW - 1) / NW) + UN - 1) / UN;
@@ -232,7 +273,7 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
             sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
             __syncthreads();
             const int K = pc.ksteps;
-            const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(a.dval + pc.data_off);
+            const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(mb.dval + pc.data_off);
             // pipeline, DEPTH k-steps deep: A of k-step s sits in register
             // slot s % DEPTH and X of k-step s in slot s % DEPTH, each loaded
             // DEPTH k-steps before its use (A is streamed from HBM, so one
@@ -285,42 +326,38 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
                         ar[sl][i][1] = __builtin_nontemporal_load(&A[off + 64]);
                     }
                 }
-                // two column tiles per step, and the pass loop outside
-                // the unit and column loops, so consecutive MFMAs go to
-                // different accumulators (no back-to-back dependence).
-                // A second column past ntw reads column j0's B and is
-                // never stored.  Active units are a prefix (uact[i]
-                // implies uact[i - 1]), so one branch picks the count.
+                // two column tiles per step: per unit, its 2 x passes MFMAs
+                // alternate between the two accumulators, so no MFMA waits on
+                // the one before it.  A second column past ntw reads column
+                // j0's B and is never stored.  A unit's branch is uniform
+                // over the wavefront (inactive units issue nothing).
                 constexpr int JB = Cfg::NTMAX >= 2 ? 2 : 1;
-                auto passes = [&](auto nu) {
-                    constexpr int NU = decltype(nu)::value;
 #pragma unroll
-                    for (int j0 = 0; j0 < Cfg::NTMAX; j0 += JB) {
-                        if (j0 < ntw) {
-                            sb_bf16x8 bt[JB][SPLIT];
+                for (int j0 = 0; j0 < Cfg::NTMAX; j0 += JB) {
+                    if (j0 < ntw) {
+                        sb_bf16x8 bt[JB][SPLIT];
 #pragma unroll
-                            for (int c = 0; c < JB; ++c) {
-                                const int n = (gj0 + (j0 + c < ntw ? j0 + c : j0)) * 16 + nb;
+                        for (int c = 0; c < JB; ++c) {
+                            const int n = (gj0 + (j0 + c < ntw ? j0 + c : j0)) * 16 + nb;
 #pragma unroll
-                                for (int s = 0; s < SPLIT; ++s)
-                                    bt[c][s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE +
-                                                                                 n * SB_KP + kb);
-                            }
-                            using Pt = SbPasses<SPLIT>;
+                            for (int s = 0; s < SPLIT; ++s)
+                                bt[c][s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE +
+                                                                             n * SB_KP + kb);
+                        }
+                        using Pt = SbPasses<SPLIT>;
+#pragma unroll
+                        for (int i = 0; i < UN; ++i) {
+                            if (!uact[i]) continue;
 #pragma unroll
                             for (int p = 0; p < Pt::N; ++p)
 #pragma unroll
-                                for (int i = 0; i < NU; ++i)
-#pragma unroll
-                                    for (int c = 0; c < JB; ++c)
-                                        if (j0 + c < Cfg::NTMAX)   // odd NTMAX: the last pair has one column
-                                            acc[i][j0 + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                                at[i][Pt::a(p)], bt[c][Pt::b(p)], acc[i][j0 + c], 0, 0, 0);
+                                for (int c = 0; c < JB; ++c)
+                                    if (j0 + c < Cfg::NTMAX)   // odd NTMAX: the last pair has one column
+                                        acc[i][j0 + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                            at[i][Pt::a(p)], bt[c][Pt::b(p)], acc[i][j0 + c], 0, 0, 0);
                         }
                     }
-                };
-                if (uact[UN - 1]) passes(SbInt<UN>{});
-                else if (uact[0]) passes(SbInt<1>{});
+                }
                 // stage k-step ks + 1 into the free buffer (after the last
                 // k-step this writes zeros nobody reads), then load the X
                 // values of k-step ks + 1 + D into the slot it frees
@@ -344,7 +381,8 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
             if constexpr (D > 3) if (ks + 2 < K) step(SbInt<(D > 3 ? 2 : 0)>{}, ks + 2);
         }
         // D fragment of column tile j: lane holds rows 16*mi + 4*(lane/16) + r,
-        // column 16*j + lane%16
+        // column 16*j + lane%16.  A chunk of a split tile stores it as its
+        // partial-sum block (unit, column tile) in that same order instead.
 #pragma unroll
         for (int i = 0; i < UN; ++i) {
             if (!uact[i]) continue;
@@ -353,14 +391,20 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
 #pragma unroll
             for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
                 if (jj < ntw) {
-                    const int col = (j0 + jj) * 16 + nb;
+                    if (split) {
+                        const int64_t blk = (int64_t)tile.part_blk + (int64_t)(tile.kpart * mt + umi[i]) * nt + j0 + jj;
+                        reinterpret_cast<sb_f32x4 *>(mb.part)[blk * 64 + lane] = acc[i][jj];
+                    } else {
+                        const int col = (j0 + jj) * 16 + nb;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (rbase + r < hs) Y[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
+                        for (int r = 0; r < 4; ++r)
+                            if (rbase + r < hs) Y[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
+                    }
                 }
             }
         }
     }
+    if (split) sb_mfma_fixup<NW>(tile, mb, Y, N, nt, mt, fixflag);
 }
 
 template <int WG, int NRHS, int XCAP, int SPLIT>
@@ -370,6 +414,7 @@ __device__ __forceinline__ void sb_spmm_mfma_body(const SbSpmmArgs &a)
     __shared__ __attribute__((aligned(16))) int xrow[XCAP + 32];   // k-steps read up to 31 past ncols
     __shared__ __attribute__((aligned(16))) unsigned char smem[Cfg::SMEM_BYTES];
     __shared__ int next_tile;
+    __shared__ int fixflag;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmm_mfma_tile<WG, NRHS, SPLIT>(a, t, xrow, smem); });
+                 [&](int t) { sb_spmm_mfma_tile<WG, NRHS, SPLIT>(a, t, xrow, smem, &fixflag); });
 }

@@ -154,7 +154,7 @@ class Plan:
     def __init__(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, routes=None, *,
                  op="spmv", n_rhs=1, host_only=False, wg=None, xcap=4095, tile_bytes=None,
                  max_seg_rows=0, min_seg_rows=0, mode=None,
-                 grid=0, unroll=None, use_rtc=True, mfma_split=3, min_tiles=0, tuned=True):
+                 grid=0, unroll=None, use_rtc=True, mfma_split=3, min_tiles=0, mfma_kchunk=0, tuned=True):
         if not host_only:
             init()
         given = {"wg": wg, "tile_bytes": tile_bytes, "mode": mode, "unroll": unroll}
@@ -177,6 +177,7 @@ class Plan:
         o.use_rtc, o.host_only = int(bool(use_rtc)), int(bool(host_only))
         o.mfma_split = int(mfma_split)
         o.min_tiles = int(min_tiles)
+        o.mfma_kchunk = int(mfma_kchunk)
         r = None
         if routes is not None:
             r = np.ascontiguousarray(routes, dtype=np.uint8)
@@ -218,7 +219,8 @@ class Plan:
                            ("piece_end", "<i4"), ("mat", "<i4"), ("dense_tpar", "<i4"),
                            ("dense_mcols", "<i4"), ("sp_glog", "<i4"),
                            ("sp_map", "<i4"), ("sp_waves", "<i4"),
-                           ("y_accum", "<i4"), ("stage_map", "<i4"), ("stage_len", "<i4")])
+                           ("y_accum", "<i4"), ("stage_map", "<i4"), ("stage_len", "<i4"),
+                           ("ksplit", "<i4"), ("kpart", "<i4"), ("fix", "<i4"), ("part_blk", "<i4")])
     PIECE_DTYPE = np.dtype([("data_off", "<i8"), ("kind", "<i4"), ("ncols", "<i4"),
                             ("xr_begin", "<i4"), ("xr_end", "<i4"), ("rowptr_off", "<i4"),
                             ("ksteps", "<i4"), ("grp_end", "<i4"), ("stage_len", "<i4"),

@@ -146,3 +146,9 @@ def test_vbr_spmm_codegen_threads_same_program(golden, tmp_path):
     with pytest.raises(ValueError):
         codegen.gen_multi_threaded_spmm(0, *read_vbr(str(tmp_path / "example.vbr")), 0, str(tmp_path), "x",
                                         str(tmp_path))
+
+
+def test_print_program_output_follows_redirected_stdout(capsys):
+    """The default stream is the current sys.stdout, not the one at import."""
+    codegen.print_program_output("m", [5], np.array([1.0], np.float32), "spmv")
+    assert capsys.readouterr().out == "m = 5,\n1.000000\n"

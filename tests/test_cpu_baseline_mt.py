@@ -64,3 +64,43 @@ def test_threads_bit_identical_to_single_thread(dtype):
     for t in (2, 5, 8):
         prog.set_threads(t)
         assert np.array_equal(prog.run(x).view(np.uint8), y1.view(np.uint8)), t
+
+
+def test_remove_outliers_deciles_known_answers():
+    """eval.py:14-21: fewer than 10 values kept as they are; else values in
+    [P10, P90] (numpy linear percentiles: 1..20 -> P10 = 2.9, P90 = 18.1)."""
+    from oracle.cpu_protocol import remove_outliers_deciles, parse_line0
+    assert remove_outliers_deciles([5, 1, 9]) == [5, 1, 9]
+    assert remove_outliers_deciles(list(range(1, 21))) == list(range(3, 19))
+    assert remove_outliers_deciles([7.0] * 12) == [7.0] * 12
+    assert parse_line0("prog = 10,20,30,") == [10.0, 20.0, 30.0]
+
+
+def test_process_protocol_runs_children(tmp_path):
+    """Three child processes of a small program (SpMV and threaded SpMM):
+    line 0 parsed, medians kept (fewer than 10 runs: no filter), mean > 0."""
+    from oracle import cpu_protocol as P
+    vbr, x = synth.workload_c1()
+    routes = threshold_routes(vbr.val, vbr.indx, 15)
+    val2, indx2, ub, ip, ind, cv = compress_with_routes(vbr.val, vbr.rpntr, vbr.cpntr, vbr.indx, vbr.bindx,
+                                                        vbr.bpntrb, vbr.bpntre, routes)
+    path = str(tmp_path / "c1.npz")
+    P.save_spmv_program(path, val2, indx2, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre, ub, ip, ind,
+                        cv, x)
+    r = P.time_processes(path, 1, 5, procs=3)
+    assert r["runs"] == 3 and r["kept_runs"] == 3 and r["mean_s"] > 0 and len(r["run_medians_s"]) == 3
+    X = np.random.default_rng(0).uniform(-1, 1, (1000, 8)).astype(np.float32)
+    path = str(tmp_path / "c1m.npz")
+    P.save_spmm_program(path, vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre, routes, X)
+    r = P.time_processes(path, 3, 2, procs=2)
+    assert r["mean_s"] > 0
+
+
+def test_threaded_spmm_bit_identical():
+    vbr, _, _ = synth.mixed_vbr(3000, 30, 32, 192, 0.05, 0.7, (0.9, 1.0), (0.02, 0.1), seed=7)
+    X = np.random.default_rng(1).uniform(-1, 1, (3000, 16)).astype(np.float32)
+    routes = threshold_routes(vbr.val, vbr.indx, 15)
+    args = (vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre, routes, X, 16)
+    Y1 = c_oracle.spmm(*args)
+    for t in (2, 5, 8):
+        assert np.array_equal(c_oracle.spmm(*args, threads=t), Y1)

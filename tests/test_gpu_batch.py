@@ -112,3 +112,30 @@ def test_group_rejects_host_only_member(rt):
     p = rt.Plan.from_vbr(vbr, None, host_only=True)
     with pytest.raises(Exception):
         rt.Group([p], [rt.DeviceArray.from_numpy(x)], [rt.DeviceArray(p.nrows)])
+
+
+def test_group_keeps_each_members_split(rt):
+    """SpMM members built with the 3-term split and with hi+lo in one group:
+    they land in different launches (the part key holds the split), and each
+    output is bit-identical to that plan's own launch."""
+    mats = _mats(4, seed=13)
+    plans, xs, ys, own = [], [], [], []
+    for j, (vbr, _, X) in enumerate(mats):
+        r = threshold_routes(vbr.val, vbr.indx, 15)
+        rm = np.where(r == ROUTE_DENSE, ROUTE_DENSE_MFMA, r).astype(np.uint8)
+        p = rt.Plan.from_vbr(vbr, rm, op="spmm", n_rhs=16, mfma_split=2 if j % 2 else 3)
+        plans.append(p)
+        xs.append(rt.DeviceArray.from_numpy(X))
+        ys.append(rt.DeviceArray((p.nrows, 16)))
+        own.append(p.spmm(X))
+    # the two splits give different results on these inputs
+    assert not all(np.array_equal(a, b) for a, b in zip(own, [q.spmm(X) for q, (_, _, X) in zip(
+        [rt.Plan.from_vbr(v, np.where(threshold_routes(v.val, v.indx, 15) == ROUTE_DENSE, ROUTE_DENSE_MFMA,
+                                      threshold_routes(v.val, v.indx, 15)).astype(np.uint8), op="spmm",
+                          n_rhs=16, mfma_split=3) for v, _, _ in mats], mats)]))
+    print("case: group of split-3 and split-2 spmm members", flush=True)
+    g = rt.Group(plans, xs, ys)
+    g.launch()
+    for y, o in zip(ys, own):
+        assert np.array_equal(y.numpy(), o)
+    g.close()

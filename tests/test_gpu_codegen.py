@@ -11,6 +11,7 @@ within 1e-5 componentwise of FP64."""
 import gzip
 import os
 import shutil
+import sys
 
 import numpy as np
 import pytest
@@ -32,6 +33,8 @@ def rt():
 
 
 def _stdout(capsys, path):
+    """The program's stdout through the CLI entry (earlier output dropped)."""
+    capsys.readouterr()
     assert codegen.main(["run", path]) == 0
     return capsys.readouterr().out.split("\n")
 
@@ -49,7 +52,7 @@ def test_spmv_flow_example_canon(rt, golden, tmp_path, capsys):
     write_dense_vector(1.0, 11)
     canon = _canon_floats(os.path.join(golden, "output_spmv_canon.txt"))
     for threads in (1, 4):
-        print(f"case: codegen spmv example threads={threads}", flush=True)
+        print(f"case: codegen spmv example threads={threads}", file=sys.stderr, flush=True)
         d = str(tmp_path / f"t{threads}")
         codegen.vbr_spmv_codegen("example", d, str(tmp_path), threads=threads, bench=5)
         lines = _stdout(capsys, os.path.join(d, "example"))
@@ -69,7 +72,7 @@ def test_spmv_flow_routed_canon(rt, kat, golden, tmp_path, capsys):
     val, indx, bindx, bb, be = convert.convert_sparse_to_vbr(sp.csc_matrix(d), rp, cp, "example2", str(tmp_path))
     convert.convert_vbr_to_compressed(val, rp, cp, indx, bindx, bb, be, "example2", str(tmp_path), density=80)
     write_dense_vector(1.0, 11)
-    print("case: codegen spmv example2 threshold 80", flush=True)
+    print("case: codegen spmv example2 threshold 80", file=sys.stderr, flush=True)
     codegen.vbr_spmv_codegen("example2", str(tmp_path), str(tmp_path), threads=1)
     lines = _stdout(capsys, str(tmp_path / "example2"))
     body = [float(l) for l in lines[1:] if l.strip()]
@@ -84,7 +87,7 @@ def test_spmm_flow_example_canon(rt, golden, tmp_path, capsys, precision, thread
     MFMA route takes at most 256, whose Y is the canon's first 256 columns)."""
     shutil.copy(os.path.join(golden, "example.vbr"), tmp_path / "example.vbr")
     write_dense_matrix(1.0, 11, n_rhs)
-    print(f"case: codegen spmm example {precision} threads={threads} N={n_rhs}", flush=True)
+    print(f"case: codegen spmm example {precision} threads={threads} N={n_rhs}", file=sys.stderr, flush=True)
     codegen.vbr_spmm_codegen("example", 0, str(tmp_path), str(tmp_path), threads=threads, n_rhs=n_rhs,
                              precision=precision)
     lines = _stdout(capsys, str(tmp_path / "example"))
@@ -102,7 +105,7 @@ def test_spmv_flow_random_x_c1(rt, tmp_path):
     y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
                                              vbr.bpntre, x)
     for T in (0, 100):
-        print(f"case: codegen spmv c1 T={T}", flush=True)
+        print(f"case: codegen spmv c1 T={T}", file=sys.stderr, flush=True)
         convert.convert_vbr_to_compressed(vbr.val, vbr.rpntr, vbr.cpntr, vbr.indx, vbr.bindx, vbr.bpntrb,
                                           vbr.bpntre, "c1", str(tmp_path), density=T)
         codegen.vbr_spmv_codegen("c1", str(tmp_path), str(tmp_path), threads=1, bench=3)
@@ -123,7 +126,7 @@ def test_spmm_flow_random_x(rt, tmp_path, n_rhs):
     X = np.random.default_rng(9).uniform(-1, 1, (3000, n_rhs)).astype(np.float32)
     Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
                                              vbr.bpntre, X)
-    print(f"case: codegen spmm mixed 3000 N={n_rhs}", flush=True)
+    print(f"case: codegen spmm mixed 3000 N={n_rhs}", file=sys.stderr, flush=True)
     codegen.vbr_spmm_codegen("m", 15, str(tmp_path), str(tmp_path), threads=1, n_rhs=n_rhs, precision="bf16x6")
     _, Y = codegen.run_program(str(tmp_path), "m", x=X)
     assert c_oracle.scaled_error(Y.ravel(), Y64.ravel(), AY.ravel())["componentwise"] <= TOL

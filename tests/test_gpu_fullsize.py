@@ -1,0 +1,196 @@
+"""Parity at the full sizes of the benchmark workloads (marked gpu).
+
+- 100,000 x 100,000 mixed SpMV (1000 x 1000 partitions sized 20-200, 1 %
+  blocks, half density 0.8-1.0 and half 0.01-0.2, seed 2): the plain plan
+  (threshold 15) and the router's plan at sparse bias 0.5.
+- 50,000 x 50,000 SpMM with N = 128 (500 x 500 partitions sized 32-192, 2 %
+  blocks, 70 % density 0.9-1.0 and 30 % 0.02-0.1, seed 3): FP32 plain, MFMA
+  with the 3-term split (6 passes) and with hi+lo (3 passes) on the
+  threshold routes, and the all-MFMA plan in 256-thread workgroups.
+- one held-out router matrix (n = 72,400, built like the 100k workload,
+  seed 4000): the keep-if-faster rule of choose_plan.
+
+Bar: componentwise-scaled error against FP64 <= 1e-5 (max_i |y_i - y64_i| /
+(|A||x|)_i); repeated launches bit-identical.  The hi+lo split has a
+worst-case bound of about 4.6e-5 (sable_amd.codegen.MFMA_SPLIT_BOUND); on
+these inputs it is asserted at the same 1e-5 bar, as a measured property of
+the workloads, not a guarantee.
+"""
+import numpy as np
+import pytest
+
+from oracle import c_oracle
+from sable_amd import synth
+from sable_amd.convert import threshold_routes, ROUTE_DENSE, ROUTE_DENSE_MFMA
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from sable_amd import runtime
+    runtime.init()
+    return runtime
+
+
+@pytest.fixture(scope="module")
+def c2():
+    vbr, x, _ = synth.workload_c2()
+    y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                             vbr.bpntre, x)
+    return vbr, x, y64, ay
+
+
+@pytest.fixture(scope="module")
+def c3():
+    vbr, X, _ = synth.workload_c3()
+    Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                             vbr.bpntre, X)
+    return vbr, X, Y64, AY
+
+
+@pytest.mark.timeout(600)
+def test_c2_plain_and_learned(rt, c2):
+    from sable_amd.router import Router
+    vbr, x, y64, ay = c2
+    assert vbr.rpntr[-1] == 100_000 and len(vbr.bindx) == 10_000
+    cases = [("plain T=15", threshold_routes(vbr.val, vbr.indx, 15))]
+    if Router.available():
+        r = Router.load()
+        cases.append(("learned sparse bias 0.5", r.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                                vbr.bpntrb, vbr.bpntre, 1, sparse_bias=0.5)))
+    xd = rt.DeviceArray.from_numpy(x)
+    yd = rt.DeviceArray(100_000)
+    for name, routes in cases:
+        print(f"case: c2 100k spmv {name}", flush=True)
+        plan = rt.Plan.from_vbr(vbr, routes)
+        plan.time(xd, yd, warmup=0, reps=1)
+        y1 = yd.numpy()
+        plan.time(xd, yd, warmup=0, reps=1, flush_bytes=1 << 30)
+        assert np.array_equal(y1, yd.numpy())
+        e = c_oracle.scaled_error(y1, y64, ay)["componentwise"]
+        print(f"  componentwise error {e:.3e}", flush=True)
+        assert e <= TOL
+        plan.close()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("variant", ["fp32_plain", "mfma_x6", "mfma_x3", "all_mfma_wg256"])
+def test_c3_spmm_variants(rt, c3, variant):
+    vbr, X, Y64, AY = c3
+    assert vbr.rpntr[-1] == 50_000 and X.shape == (50_000, 128)
+    base = threshold_routes(vbr.val, vbr.indx, 15)
+    mf = np.where(base == ROUTE_DENSE, ROUTE_DENSE_MFMA, base).astype(np.uint8)
+    routes, split, kw = {"fp32_plain": (base, 3, {}), "mfma_x6": (mf, 3, {}), "mfma_x3": (mf, 2, {}),
+                         "all_mfma_wg256": (np.full(len(base), ROUTE_DENSE_MFMA, np.uint8), 3,
+                                            {"wg": 256})}[variant]
+    print(f"case: c3 50k spmm N=128 {variant}", flush=True)
+    plan = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=128, mfma_split=split, **kw)
+    if variant == "all_mfma_wg256":
+        assert plan.stats["nblocks_mfma"] == len(base) and plan.stats["wg"] == 256
+    Xd = rt.DeviceArray.from_numpy(X)
+    Yd = rt.DeviceArray((50_000, 128))
+    plan.time(Xd, Yd, warmup=0, reps=1)
+    Y1 = Yd.numpy()
+    plan.time(Xd, Yd, warmup=0, reps=1, flush_bytes=1 << 30)
+    assert np.array_equal(Y1, Yd.numpy())
+    e = c_oracle.scaled_error(Y1.ravel(), Y64.ravel(), AY.ravel())["componentwise"]
+    print(f"  componentwise error {e:.3e}", flush=True)
+    assert e <= TOL
+    plan.close()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n_rhs", [16, 128])
+def test_hi_lo_split_at_the_workload_widths(rt, n_rhs):
+    """hi+lo (3 passes) against FP64 at the widths the SpMM workloads use
+    (N = 128 single matrix, N = 16 batch) on a 20k matrix of the 50k kind."""
+    vbr, _, _ = synth.mixed_vbr(20_000, 200, 32, 192, 0.02, 0.7, (0.9, 1.0), (0.02, 0.1), seed=31)
+    X = np.random.default_rng(32).uniform(-1, 1, (20_000, n_rhs)).astype(np.float32)
+    Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                             vbr.bpntre, X)
+    routes = np.full(len(vbr.bindx), ROUTE_DENSE_MFMA, np.uint8)
+    print(f"case: hi+lo 20k N={n_rhs}", flush=True)
+    Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs, mfma_split=2).spmm(X)
+    e = c_oracle.scaled_error(Y.ravel(), Y64.ravel(), AY.ravel())["componentwise"]
+    print(f"  componentwise error {e:.3e}", flush=True)
+    assert e <= TOL
+
+
+@pytest.mark.timeout(600)
+def test_router_held_out_matrix_keep_if_faster(rt):
+    """The first held-out matrix of the router evaluation (n = 72,400, seed
+    4000): choose_plan keeps the learned plan only when it is faster at the
+    same tolerance, and the kept plan is within 1e-5."""
+    from sable_amd.router import Router
+    from sable_amd.router.router import choose_plan
+    if not Router.available():
+        pytest.skip("no router weights")
+    rng = np.random.default_rng(4000)
+    n = int(rng.integers(20_000, 80_001))
+    n -= n % 100
+    assert n == 72_400
+    vbr, x, _ = synth.workload_like_c2(n, seed=4000)
+    print("case: router held-out n=72400", flush=True)
+    plan, rep = choose_plan(vbr, x, router=Router.load(), reps=10)
+    print(f"  {rep}", flush=True)
+    assert rep["kept"] in ("fixed", "learned")
+    if rep["kept"] == "learned":
+        assert rep["learned_ms"] < rep["fixed_ms"]
+    else:
+        assert not rep.get("learned_within_tol") or rep["learned_ms"] >= rep["fixed_ms"]
+    y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                             vbr.bpntre, x)
+    assert c_oracle.scaled_error(plan.spmv(x), y64, ay)["componentwise"] <= TOL
+
+
+@pytest.mark.timeout(1500)
+def test_c5_group_of_256_matrices(rt):
+    """The 256-matrix batch (n log-uniform 2,000-64,000, 50-800 partitions,
+    1 % blocks, seed 5), each matrix with an SpMV and an SpMM with N = 16,
+    all 512 plans in one group (one hipGraph): after every replay each
+    output is bit-identical to the member plan's own launch; a fixed sample
+    of 8 matrices is checked against FP64."""
+    from sable_amd.router import Router
+    router = Router.load() if Router.available() else None
+    shapes = synth.c5_shapes(256)
+    plans, xs, ys_g, ys_m, samples = [], [], [], [], []
+    for i, (n, parts) in enumerate(shapes):
+        vbr, x, X = synth.workload_c5_matrix(i, n, parts, n_rhs=16)
+        fixed = threshold_routes(vbr.val, vbr.indx, 15)
+        for op, inp in (("spmv", x), ("spmm", X)):
+            k = 1 if op == "spmv" else 16
+            routes = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                          vbr.bpntre, k, mfma_bias=1.0 if k > 1 else 0.0) if router else fixed)
+            p = rt.Plan.from_vbr(vbr, routes, op=op, n_rhs=k, min_tiles=1, wg=128 if k > 1 else 256)
+            shape = (n,) if k == 1 else (n, k)
+            xd = rt.DeviceArray.from_numpy(inp)
+            ym = rt.DeviceArray(shape)
+            p.time(xd, ym, warmup=0, reps=1)           # the member's own launch
+            plans.append(p)
+            xs.append(xd)
+            ys_m.append(ym)
+            ys_g.append(rt.DeviceArray.zeros(shape))
+        if i % 32 == 0:
+            samples.append((i, vbr, x, X))
+        if (i + 1) % 64 == 0:
+            print(f"case: c5 {i + 1}/256 matrices planned", flush=True)
+    print("case: c5 group of 512 plans", flush=True)
+    g = rt.Group(plans, xs, ys_g)
+    own = [y.numpy() for y in ys_m]
+    for rep in range(2):
+        g.launch()
+        diff = [j for j, (a, b) in enumerate(zip(ys_g, own)) if not np.array_equal(a.numpy(), b)]
+        assert diff == [], f"replay {rep}: members {diff[:8]} differ from their own launches"
+        plans[0].time(xs[0], ys_m[0], warmup=0, reps=1)   # eager work between replays
+    for i, vbr, x, X in samples:
+        y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                 vbr.bpntrb, vbr.bpntre, x)
+        assert c_oracle.scaled_error(ys_g[2 * i].numpy(), y64, ay)["componentwise"] <= TOL
+        Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                 vbr.bpntrb, vbr.bpntre, X)
+        assert c_oracle.scaled_error(ys_g[2 * i + 1].numpy().ravel(), Y64.ravel(), AY.ravel())[
+            "componentwise"] <= TOL
+    g.close()
