@@ -147,7 +147,7 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
     the FP32 vector path; MFMA = the same routes with dense blocks on the
     bf16-split MFMA path (3-term split, 6 passes; and hi+lo, 3 passes);
     learned = the router's plans at MFMA-route biases 0, 0.5 and 1 (3-term
-    split), each in 512- and 256-thread workgroups.  The fastest learned
+    split), each in the workgroup size the router picks.  The fastest learned
     plan within 1e-5 of FP64 is kept if faster than MFMA x6 on the threshold
     routes; the kept plan is then timed again with fresh launches.  Returns
     the result dict (roofline of the kept plan, every variant's time and
@@ -168,10 +168,10 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
     if args.routing in ("learned", "both") and Router.available():
         router = Router.load()
         for sb, mb in BIAS_GRID["spmm"]:
-            lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                     vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
-            for wg in (512, 256):
-                variants.append((f"learned_x6 mfma_bias={mb:g} wg={wg}", lr, 3, {"wg": wg, "mfma_bias": mb}))
+            # the router's routes and tile shape (workgroup size)
+            lr, _, wg = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                               vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
+            variants.append((f"learned_x6 mfma_bias={mb:g} wg={wg}", lr, 3, {"wg": wg, "mfma_bias": mb}))
     plans = {}
     for name, routes, split, kw in variants:
         kw = dict(kw)
@@ -309,16 +309,17 @@ def bench_batch(args, runtime, info):
             shape = (n,) if op == "spmv" else (n, n_rhs)
             pp = runtime.Plan.from_vbr(vbr, fixed, op=op, n_rhs=k)
             for j, (sb, mb) in enumerate(biases):
-                lr = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                          vbr.bpntre, k, sparse_bias=sb if k == 1 else 0.0,
-                                          mfma_bias=mb if k > 1 else 0.0) if router else fixed)
                 # built for the group: no per-plan minimum tile count (the group
-                # fills the GPU; small matrices keep block-row-sized tiles), and
-                # smaller workgroups than the 100k tuning for these short tiles:
-                # SpMV 256 threads, SpMM N = 16 128 threads (the group measured
-                # 3.10 ms so, 3.46 with SpMV at 512, 3.65 with SpMM at 256 too)
-                pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1,
-                                           wg=128 if op == "spmm" else 256)
+                # fills the GPU; small matrices keep block-row-sized tiles), all
+                # members of an operation in the router's narrow tile shape
+                # (one launch per operation; SpMV 256 threads, SpMM N = 16 128
+                # threads: the group measured 3.10 ms so, 3.46 with SpMV at 512,
+                # 3.65 with SpMM at 256 too), routed for that shape
+                lr, _, wgl = (router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                                     vbr.bpntre, k, sparse_bias=sb if k == 1 else 0.0,
+                                                     mfma_bias=mb if k > 1 else 0.0, shape=0)
+                              if router else (fixed, 0, 128 if op == "spmm" else 256))
+                pl = runtime.Plan.from_vbr(vbr, lr, op=op, n_rhs=k, min_tiles=1, wg=wgl)
                 cand[j].append(pl)
                 cand_bytes[j] += pl.stats["bytes_alg"]
             plain.append(pp)
@@ -737,9 +738,9 @@ def main():
                                              vbr.bpntrb, vbr.bpntre, x)
     plans = {}
 
-    def measure(routes, label):
+    def measure(routes, label, wg=None):
         t1 = time.time()
-        plan = runtime.Plan.from_vbr(vbr, routes, **opts)
+        plan = runtime.Plan.from_vbr(vbr, routes, **dict(opts, **({"wg": wg} if wg else {})))
         inspector_s = time.time() - t1
         st = plan.stats
         log(f"{label} plan: {st['ntiles']} tiles, rtc={st['rtc_used']}, grid={st['grid']}, "
@@ -767,9 +768,9 @@ def main():
             # one learned plan per sparse-route bias; the fastest within 1e-5 is "learned"
             cands = []
             for sb, mb in BIAS_GRID["spmv"]:
-                lr = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                         vbr.bpntre, 1, sparse_bias=sb, mfma_bias=mb)
-                r = measure(lr, f"learned sparse_bias={sb:g}")
+                lr, _, wgl = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                    vbr.bpntrb, vbr.bpntre, 1, sparse_bias=sb, mfma_bias=mb)
+                r = measure(lr, f"learned sparse_bias={sb:g}", wg=wgl)
                 r["sparse_bias"] = sb
                 r["routes_changed"] = int(np.sum(lr != routes))
                 r["label"] = f"learned sparse_bias={sb:g}"

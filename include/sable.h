@@ -92,8 +92,8 @@ typedef struct {
                                  SpMV, 256 for SpMM */
     int32_t mfma_kchunk;      /* SpMM MFMA tiles: at most about this many 32-column
                                  k-steps per work item; a longer row tile is split
-                                 along k into chunks whose partial sums the last
-                                 chunk to finish adds in chunk order (deterministic);
+                                 along k into chunks whose partial sums a fixup
+                                 launch adds in chunk order (deterministic);
                                  0 = 32, negative = no split */
 } SablePlanOpts;
 
@@ -120,8 +120,8 @@ typedef struct {
     int32_t rtc_used;         /* 1 if the launch kernel came from hipRTC */
     int32_t mode, grid, wg;
     int32_t ntiles_split;     /* SpMM: MFMA row tiles split along k */
-    int64_t scratch_bytes;    /* device scratch of the split tiles: partial sums and
-                                 one counter each (not counted in bytes_alg) */
+    int64_t scratch_bytes;    /* device scratch of the split tiles' partial sums (not
+                                 counted in bytes_alg) */
     int64_t fixup_bytes;      /* partial sums written and read back per launch */
 } SablePlanStats;
 

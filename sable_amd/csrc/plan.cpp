@@ -470,7 +470,6 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             t.y_accum = y_accum;
             t.ksplit = 1;
             t.kpart = 0;
-            t.fix = -1;
             t.part_blk = 0;
             // sparse rows of the segment, per sparse column group
             std::vector<SparseRows> sp;
@@ -655,7 +654,8 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                     continue;
                 }
                 // one work item per k chunk; scratch for nc partials of
-                // (16-row units x N/16 column tiles) 1 KiB blocks, one counter
+                // (16-row units x N/16 column tiles) 1 KiB blocks; the row
+                // tile itself goes to the fixup list
                 const int64_t blocks = (int64_t)((hs + 15) / 16) * (P.n_rhs / 16);
                 if (P.part_blocks + nc * blocks > INT32_MAX) {
                     err = "partial-sum scratch exceeds 2^31 KiB";
@@ -666,11 +666,15 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                     SbTile &tt = P.tiles.back();
                     tt.ksplit = nc;
                     tt.kpart = c;
-                    tt.fix = P.nfix;
                     tt.part_blk = (int32_t)P.part_blocks;
                 }
+                SbTile ft{};
+                ft.row0 = r0 + st[s];
+                ft.nrows = hs;
+                ft.ksplit = nc;
+                ft.part_blk = (int32_t)P.part_blocks;
+                P.fix_tiles.push_back(ft);
                 P.part_blocks += nc * blocks;
-                P.nfix += 1;
                 S.ntiles_split += 1;
                 S.fixup_bytes += (2 * nc) * blocks * 1024;
             }
@@ -718,7 +722,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     S.bytes_alg = 4 * S.dense_values + 4 * S.mfma_values + 6 * S.sparse_nnz + 4 * S.rowptr_entries +
                   S.descriptor_bytes + 4LL * P.ncols * P.n_rhs + 4LL * P.nrows * P.n_rhs;
     S.flops_alg = 2 * S.nnz * (int64_t)P.n_rhs;
-    S.scratch_bytes = P.part_blocks * 1024 + 4LL * P.nfix;
+    S.scratch_bytes = P.part_blocks * 1024;
     S.wg = o.wg;
     S.mode = o.mode;
     return SABLE_OK;

@@ -26,7 +26,7 @@ struct HostPlan {
     std::vector<double> tile_cost;   // estimated bytes per tile (ordering)
     std::vector<int32_t> tile_phase; // 0: SpMM MFMA tile (first launch), 1: the rest
     int64_t part_blocks = 0;         // 1 KiB partial-sum blocks of the split MFMA tiles
-    int32_t nfix = 0;                // counters of the split MFMA tiles
+    std::vector<SbTile> fix_tiles;   // one per split MFMA row tile: its rows, ksplit, part_blk
     SablePlanStats stats{};
 };
 

@@ -46,14 +46,21 @@ extern "C" __global__ void __launch_bounds__(256) sb_spmm_f32_aot(SbSpmmArgs a)
     sb_spmm_f32_body<256, 8, 0, 4096>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(256) sb_spmm_mfma_aot(SbSpmmArgs a)
+extern "C" __global__ void __launch_bounds__(256, SB_MFMA_WAVES) sb_spmm_mfma_aot(SbSpmmArgs a)
 {
     sb_spmm_mfma_body<256, 0, 4096, 3>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(256) sb_spmm_mfma2_aot(SbSpmmArgs a)
+extern "C" __global__ void __launch_bounds__(256, SB_MFMA_WAVES) sb_spmm_mfma2_aot(SbSpmmArgs a)
 {
     sb_spmm_mfma_body<256, 0, 4096, 2>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(256) sb_mfma_fixup(const SbTile *fix_tiles, int nfix,
+                                                                 const SbMatBufs *mats, const float *part, float *y,
+                                                                 int n_rhs)
+{
+    sb_mfma_fixup_body<256>(fix_tiles, nfix, mats, part, y, n_rhs);
 }
 
 extern "C" __global__ void sb_fill_u32(uint32_t *p, int64_t n, uint32_t v)
@@ -258,6 +265,9 @@ struct Spec {
     int wg = 256, unroll = 4, xcap = 4096, nrhs = 0, rtmax = 8, split = 3;
 };
 
+// The macro block of a specialisation.  SABLE_RTC_DEFINES (e.g.
+// "SB_MFMA_DEPTH=3;SB_MFMA_XDEPTH=1") adds kernel tuning macros for
+// experiments; they are part of the cache key like the rest.
 std::string spec_defines(const Spec &s)
 {
     char b[320];
@@ -265,7 +275,22 @@ std::string spec_defines(const Spec &s)
              "#define SB_WG %d\n#define SB_UNROLL %d\n#define SB_XCAP %d\n#define SB_NRHS %d\n#define SB_RTMAX %d\n"
              "#define SB_SPLIT %d\n",
              s.wg, s.unroll, s.xcap, s.nrhs, s.rtmax, s.split);
-    return b;
+    std::string d = b;
+    if (const char *extra = getenv("SABLE_RTC_DEFINES")) {
+        std::string e = extra;
+        size_t i = 0;
+        while (i < e.size()) {
+            size_t j = e.find(';', i);
+            if (j == std::string::npos) j = e.size();
+            std::string item = e.substr(i, j - i);
+            const size_t eq = item.find('=');
+            if (!item.empty() && item.find_first_not_of("ABCDEFGHIJKLMNOPQRSTUVWXYZ_0123456789=") == std::string::npos &&
+                eq != std::string::npos)
+                d += "#define " + item.substr(0, eq) + " " + item.substr(eq + 1) + "\n";
+            i = j + 1;
+        }
+    }
+    return d;
 }
 
 const char *kStubSpmv =
@@ -275,7 +300,7 @@ const char *kStubSpmmF32 =
     "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_f32_rtc(SbSpmmArgs a)\n"
     "{ sb_spmm_f32_body<SB_WG, SB_RTMAX, SB_NRHS, SB_XCAP>(a); }\n";
 const char *kStubSpmmMfma =
-    "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_mfma_rtc(SbSpmmArgs a)\n"
+    "extern \"C\" __global__ void __launch_bounds__(SB_WG, SB_MFMA_WAVES) sb_spmm_mfma_rtc(SbSpmmArgs a)\n"
     "{ sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP, SB_SPLIT>(a); }\n";
 
 int pow2_at_least(int v)
@@ -321,10 +346,10 @@ struct SablePlan {
     int32_t *d_srowptr = nullptr;
     int32_t *d_xmap = nullptr;
     int32_t *d_queue = nullptr;
-    // scratch of the MFMA tiles split along k: partial sums, counters (zero
-    // between launches: the last chunk of a tile resets its counter)
+    // MFMA tiles split along k: partial-sum scratch and the fixup list
     float *d_part = nullptr;
-    int32_t *d_fix = nullptr;
+    SbTile *d_fix_tiles = nullptr;
+    int32_t nfix = 0;
     // host staging for SABLE_HOST_PTRS
     float *d_xtmp = nullptr, *d_ytmp = nullptr;
     int64_t xtmp_n = 0, ytmp_n = 0;
@@ -348,6 +373,9 @@ struct GroupPart {
     SbXRange *d_xr = nullptr;
     SbMatBufs *d_mats = nullptr;
     int32_t *d_queue = nullptr;
+    SbTile *d_fix_tiles = nullptr;   // the members' split MFMA row tiles (matrix slot in .mat)
+    int32_t nfix = 0;
+    int32_t ntiles_mfma = 0;         // MFMA tiles (the first phase when nonzero)
     std::vector<Phase> phases;
 };
 
@@ -508,7 +536,7 @@ int launch_raw(bool rtc, hipFunction_t fn, const void *aot, int grid, int wg, vo
 int launch_phase(int op, int n_rhs, const Phase &ph, int wg, int mode, const SbTile *tiles,
                  const SbPiece *pieces, const SbXRange *xr, const float *dval, const float *sval,
                  const uint16_t *sidx, const int32_t *srowptr, const int32_t *xmap, int32_t *queue,
-                 const SbMatBufs *mats, const float *x, float *y, float *part, int32_t *fix, hipStream_t st,
+                 const SbMatBufs *mats, const float *x, float *y, float *part, hipStream_t st,
                  hipEvent_t e0, hipEvent_t e1)
 {
     const int n = ph.t1 - ph.t0;
@@ -525,27 +553,48 @@ int launch_phase(int op, int n_rhs, const Phase &ph, int wg, int mode, const SbT
     SbSpmmArgs a{};
     a.tiles = tiles + ph.t0; a.pieces = pieces; a.xr = xr;
     a.dval = dval; a.sval = sval; a.sidx = sidx; a.srowptr = srowptr;
-    a.mats = mats; a.x0 = x; a.y0 = y; a.queue = queue; a.part = part; a.fix = fix;
+    a.mats = mats; a.x0 = x; a.y0 = y; a.queue = queue; a.part = part;
     a.ntiles = n; a.mode = mode; a.n_rhs = n_rhs;
     return launch_raw(ph.rtc, ph.fn, ph.aot, grid, wg, &a, st, e0, e1);
 }
 
-// Launches every phase of a plan in order on its stream.  With events: a
-// single launch carries them itself (kernel start and end); several
-// launches are bracketed by stream events (their gaps included).
-int plan_launch(SablePlan *p, const float *x, float *y, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+// Launches the fixup of a plan's (or group part's) split MFMA tiles: one
+// workgroup per split row tile.
+int launch_fixup(const SbTile *fix_tiles, int nfix, const SbMatBufs *mats, const float *part, float *y, int n_rhs,
+                 hipStream_t st)
+{
+    if (nfix <= 0) return SABLE_OK;
+    hipLaunchKernelGGL(sb_mfma_fixup, dim3(nfix), dim3(256), 0, st, fix_tiles, nfix, mats, part, y, n_rhs);
+    HIPOK(hipGetLastError());
+    return SABLE_OK;
+}
+
+// Launches every phase of a plan in order on its stream (the fixup of split
+// MFMA tiles right after the MFMA phase), with the partial-sum scratch
+// `part`.  With events: a single launch carries them itself (kernel start
+// and end); several launches are bracketed by stream events (their gaps
+// included).
+int plan_launch_with(SablePlan *p, const float *x, float *y, float *part, hipEvent_t e0 = nullptr,
+                     hipEvent_t e1 = nullptr)
 {
     const size_t np = p->stats.ntiles > 0 ? p->phases.size() : 0;
-    const bool attached = np == 1;
+    const bool attached = np == 1 && p->nfix == 0;
     if (e0 && !attached) HIPOK(hipEventRecord(e0, p->stream));
     for (size_t i = 0; i < np; ++i) {
         const Phase &ph = p->phases[i];
         CHECKRC(launch_phase(p->host.op, p->host.n_rhs, ph, p->opts.wg, p->opts.mode, p->d_tiles, p->d_pieces,
                              p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap, p->d_queue, nullptr,
-                             x, y, p->d_part, p->d_fix, p->stream, attached ? e0 : nullptr, attached ? e1 : nullptr));
+                             x, y, part, p->stream, attached ? e0 : nullptr, attached ? e1 : nullptr));
+        if (i == 0 && p->stats.ntiles_mfma > 0)
+            CHECKRC(launch_fixup(p->d_fix_tiles, p->nfix, nullptr, part, y, p->host.n_rhs, p->stream));
     }
     if (e1 && !attached) HIPOK(hipEventRecord(e1, p->stream));
     return SABLE_OK;
+}
+
+int plan_launch(SablePlan *p, const float *x, float *y, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+{
+    return plan_launch_with(p, x, y, p->d_part, e0, e1);
 }
 
 int ensure_flush(uint32_t **buf, int64_t *n, int64_t bytes)
@@ -676,11 +725,11 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
     CHECKRC(upload(&p->d_xmap, h.xmap, bytes));
     HIPOK(hipMalloc((void **)&p->d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(p->d_queue, 0, 2 * sizeof(int32_t)));
-    if (h.nfix > 0) {
+    if (!h.fix_tiles.empty()) {
         HIPOK(hipMalloc((void **)&p->d_part, (size_t)h.part_blocks * 1024));
-        HIPOK(hipMalloc((void **)&p->d_fix, (size_t)h.nfix * sizeof(int32_t)));
-        HIPOK(hipMemset(p->d_fix, 0, (size_t)h.nfix * sizeof(int32_t)));
-        bytes += h.part_blocks * 1024 + 4LL * h.nfix;
+        CHECKRC(upload(&p->d_fix_tiles, h.fix_tiles, bytes));
+        p->nfix = (int32_t)h.fix_tiles.size();
+        bytes += h.part_blocks * 1024;
     }
     HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     p->own_stream = true;
@@ -708,7 +757,7 @@ int sable_plan_destroy(SablePlan *p)
     if (!p) return SABLE_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
     void *bufs[] = {p->d_tiles, p->d_pieces, p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap,
-                    p->d_queue, p->d_part, p->d_fix, p->d_xtmp, p->d_ytmp, p->d_flush};
+                    p->d_queue, p->d_part, p->d_fix_tiles, p->d_xtmp, p->d_ytmp, p->d_flush};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
@@ -854,7 +903,7 @@ namespace {
 
 void free_part(GroupPart &q)
 {
-    void *bufs[] = {q.d_tiles, q.d_pieces, q.d_xr, q.d_mats, q.d_queue};
+    void *bufs[] = {q.d_tiles, q.d_pieces, q.d_xr, q.d_mats, q.d_queue, q.d_fix_tiles};
     for (void *b : bufs)
         if (b) hipFree(b);
     q = GroupPart();
@@ -878,6 +927,7 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     std::vector<double> cost;
     std::vector<int32_t> phase;
     std::vector<SbMatBufs> mats;
+    std::vector<SbTile> fixes;
     for (size_t m = 0; m < idx.size(); ++m) {
         SablePlan *p = plans[idx[m]];
         if (pieces.size() + p->host.pieces.size() > (size_t)INT32_MAX ||
@@ -909,8 +959,11 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
         b.x = x[idx[m]];
         b.y = y[idx[m]];
         b.part = p->d_part;
-        b.fix = p->d_fix;
         mats.push_back(b);
+        for (SbTile ft : p->host.fix_tiles) {
+            ft.mat = (int32_t)m;
+            fixes.push_back(ft);
+        }
     }
     // global LPT order: MFMA tiles first (first launch), largest first in each phase
     std::vector<int32_t> order(tiles.size());
@@ -921,6 +974,7 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     });
     int32_t nm = 0;
     for (int32_t ph : phase) nm += (ph == 0);
+    q.ntiles_mfma = nm;
     std::vector<SbTile> sorted(tiles.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
     q.ntiles = (int32_t)sorted.size();
@@ -929,6 +983,8 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     CHECKRC(upload(&q.d_pieces, pieces, bytes));
     CHECKRC(upload(&q.d_xr, xr, bytes));
     CHECKRC(upload(&q.d_mats, mats, bytes));
+    CHECKRC(upload(&q.d_fix_tiles, fixes, bytes));
+    q.nfix = (int32_t)fixes.size();
     HIPOK(hipMalloc((void **)&q.d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(q.d_queue, 0, 2 * sizeof(int32_t)));
     // kernel: the specialisation machinery over a host plan view of the part
@@ -945,10 +1001,13 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
 
 int launch_part(const GroupPart &q, hipStream_t st)
 {
-    for (const Phase &ph : q.phases)
-        CHECKRC(launch_phase(q.op, q.n_rhs, ph, q.wg, q.mode, q.d_tiles, q.d_pieces, q.d_xr, nullptr, nullptr, nullptr,
-                             nullptr, nullptr, q.d_queue, q.d_mats, nullptr, nullptr, nullptr, nullptr, st, nullptr,
+    const bool has_mfma_phase = q.ntiles_mfma > 0;
+    for (size_t i = 0; i < q.phases.size(); ++i) {
+        CHECKRC(launch_phase(q.op, q.n_rhs, q.phases[i], q.wg, q.mode, q.d_tiles, q.d_pieces, q.d_xr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, q.d_queue, q.d_mats, nullptr, nullptr, nullptr, st, nullptr,
                              nullptr));
+        if (i == 0 && has_mfma_phase) CHECKRC(launch_fixup(q.d_fix_tiles, q.nfix, q.d_mats, nullptr, nullptr, q.n_rhs, st));
+    }
     return SABLE_OK;
 }
 
@@ -962,11 +1021,11 @@ int sable_group_create(SablePlan *const *plans, int nplans, const float *const *
     if (g_device < 0) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
     for (int i = 0; i < nplans; ++i)
         if (!plans[i] || plans[i]->host_only) return fail(SABLE_ERR_INVALID, "group member must be a device plan");
-    // a plan's split MFMA tiles own one set of partial sums and counters, so
-    // a plan listed twice would have two launches share them
+    // a plan's split MFMA tiles own one set of partial sums, so a plan
+    // listed twice would have its two copies' chunks share them
     for (int i = 0; i < nplans; ++i)
         for (int j = 0; j < i; ++j)
-            if (plans[i] == plans[j] && plans[i]->d_fix)
+            if (plans[i] == plans[j] && plans[i]->d_part)
                 return fail(SABLE_ERR_INVALID, "a plan with split MFMA tiles is listed twice in the group");
     std::unique_ptr<SableGroup> g(new SableGroup());
     // members sharing every option of the kernel specialisation form one

@@ -39,12 +39,11 @@ struct SbTile {
     int32_t stage_len;       // (copies, so its map loads follow the tile read directly)
     // SpMM MFMA row tiles split along k: this work item is chunk kpart of
     // ksplit (1 = not split).  Every chunk writes its partial sums to scratch
-    // block part_blk + kpart * (16-row units * N/16) (1 KiB blocks, in
-    // accumulator order) and counts itself in counter fix; the last to
-    // arrive adds the chunks in chunk order and writes the rows.
+    // blocks part_blk + kpart * (16-row units * N/16) (1 KiB blocks, in
+    // accumulator order); a fixup launch after the MFMA launch adds the
+    // chunks of each split row tile in chunk order and writes its rows.
     int32_t ksplit;
     int32_t kpart;
-    int32_t fix;
     int32_t part_blk;
 };
 

@@ -47,7 +47,6 @@ struct SbMatBufs {
     const float *x;               // x (SpMV) or X [ncols x n_rhs] (SpMM)
     float *y;                     // y (SpMV) or Y [nrows x n_rhs] (SpMM)
     float *part;                  // SpMM: partial sums of the split MFMA tiles
-    int32_t *fix;                 // SpMM: one arrival counter per split MFMA tile (zero between launches)
 };
 
 struct SbSpmvArgs {
@@ -63,7 +62,6 @@ struct SbSpmvArgs {
     const float *x0;              // single plan: x
     float *y0;                    // single plan: y
     float *part;                  // unused (SpMM only)
-    int32_t *fix;
     int32_t *queue;               // [0] next tile, [1] retired workgroups
     int32_t ntiles;
     int32_t mode;
@@ -81,7 +79,6 @@ struct SbSpmmArgs {
     const float *x0;              // single plan: X
     float *y0;                    // single plan: Y
     float *part;                  // single plan: partial sums of the split MFMA tiles
-    int32_t *fix;                 // single plan: their arrival counters
     int32_t *queue;
     int32_t ntiles;
     int32_t mode;
@@ -102,7 +99,6 @@ __device__ __forceinline__ SbMatBufs sb_mat_bufs(const Args &a, int mat, const i
     b.x = a.x0;
     b.y = a.y0;
     b.part = a.part;
-    b.fix = a.fix;
     return b;
 }
 

@@ -38,9 +38,24 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 #define SB_SPLIT 3
 #endif
 
-// k-steps of A and X loads in flight ahead of the MFMAs
+// k-steps of A loads in flight ahead of the MFMAs (A streams from HBM)
 #ifndef SB_MFMA_DEPTH
 #define SB_MFMA_DEPTH 2     // 1 to 4
+#endif
+// k-steps of X loads in flight ahead of their staging (X rows are shared by
+// many tiles and mostly come from L2): divides SB_MFMA_DEPTH
+#ifndef SB_MFMA_XDEPTH
+#define SB_MFMA_XDEPTH 1
+#endif
+// 1: the issue order of a k-step is set explicitly (MFMAs interleaved with
+// the splits' VALU work); 0: the compiler's schedule
+#ifndef SB_MFMA_INTERLEAVE
+#define SB_MFMA_INTERLEAVE 0
+#endif
+// waves per SIMD the MFMA kernels are compiled for: two (256 registers per
+// lane, accumulators included), so two 256-thread workgroups share a CU
+#ifndef SB_MFMA_WAVES
+#define SB_MFMA_WAVES 2
 #endif
 
 __device__ __forceinline__ sb_bf16x8 sb_as_bf16x8(sb_u4 v) { return __builtin_bit_cast(sb_bf16x8, v); }
@@ -81,9 +96,10 @@ __device__ __forceinline__ void sb_split8(const float *f, sb_bf16x8 *t)
 // the rows past the piece, splits the values and writes B[s][n * SB_KP + k]
 // (k < 32, s < SPLIT) to LDS.  Unit u of the k-step is (k group kg = u / N
 // of 8 rows, column n = u % N); its 8 X rows come from xrow in two 16-byte
-// LDS reads, and every load is unconditional (rows past the piece read
-// column n of X row 0), so there is no divergent control flow and no use of
-// a loaded value before sb_store_b.  X is read through a global-address
+// LDS reads, and every load is unconditional (xrow holds row 0 for the
+// columns past the piece, up to the next multiple of 32), so there is no
+// divergent control flow, no bounds test in the address and no use of a
+// loaded value before sb_store_b.  X is read through a global-address
 // pointer with 32-bit offsets.
 typedef const __attribute__((address_space(1))) float *sb_gfloat;
 
@@ -91,14 +107,14 @@ template <int WG, int UPT, int NRHS>
 __device__ __forceinline__ void sb_fetch_b(const float *X, int N, const int *xrow, int ncols, int ks,
                                            float (*f)[8])
 {
+    (void)ncols;   // xrow holds row 0 for the columns past the piece (sb_expand_xrow_mfma)
     const sb_gfloat gX = (sb_gfloat)X;
     constexpr bool EXACT = NRHS != 0 && UPT * WG == NRHS * 4;   // every unit in range
     const int units = N * 4;
 #pragma unroll
     for (int uu = 0; uu < UPT; ++uu) {
         const int u = threadIdx.x + uu * WG;
-        const bool uok = EXACT || u < units;
-        const int uc = uok ? u : 0;
+        const int uc = EXACT || u < units ? u : 0;
         const int kg = uc / N;
         const int n = uc - kg * N;
         const int k0 = ks * 32 + kg * 8;
@@ -106,12 +122,19 @@ __device__ __forceinline__ void sb_fetch_b(const float *X, int N, const int *xro
         const int4 rb = *reinterpret_cast<const int4 *>(xrow + k0 + 4);
         const int rows[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const bool ok = uok && k0 + j < ncols;
-            const uint32_t off = (uint32_t)(ok ? rows[j] : 0) * (uint32_t)N + (uint32_t)n;
-            f[uu][j] = gX[off];
-        }
+        for (int j = 0; j < 8; ++j) f[uu][j] = gX[(uint32_t)rows[j] * (uint32_t)N + (uint32_t)n];
     }
+}
+
+// The X row of every staged column of an MFMA piece, then row 0 for the
+// columns up to the next multiple of 32 (the last k-step's fetches read it;
+// sb_store_b zeroes those values).
+__device__ __forceinline__ void sb_expand_xrow_mfma(const SbXRange *__restrict__ xr, int r0, int r1, int ncols,
+                                                    int *xrow)
+{
+    sb_expand_xrow(xr, r0, r1, xrow);
+    const int pad = ((ncols + 31) & ~31) - ncols;
+    for (int i = threadIdx.x; i < pad; i += blockDim.x) xrow[ncols + i] = 0;
 }
 
 template <int WG, int UPT, int SPLIT, int NRHS>
@@ -166,34 +189,28 @@ struct SbMfmaCfg {
     static constexpr int SMEM_BYTES = 2 * BUF_STRIDE * 2;          // double-buffered
 };
 
-// The end of a chunk of a split MFMA tile (the chunks of one row tile are
-// separate work items over consecutive k ranges).  Every chunk has stored
-// its partial sums; it makes them visible device-wide (release fence), then
-// counts itself in the tile's counter.  The chunk that arrives last (any
-// chunk: the order of arrival does not matter) makes the others' stores
-// visible to itself (acquire fence), adds the partials in chunk order
-// 0, 1, ..., ksplit-1 -- so Y is the same whichever chunk finishes last --,
-// writes the rows and resets the counter for the next launch.  No wave waits
-// on another workgroup.
-template <int NW>
-__device__ __forceinline__ void sb_mfma_fixup(const SbTile &tile, const SbMatBufs &mb, float *Y, int N, int nt,
-                                              int mt, int *fixflag)
+// Fixup of the MFMA row tiles split along k (a launch after the MFMA
+// launch): workgroup t takes fix tile t, whose ksplit chunks stored their
+// partial sums as (16-row unit, column tile) 1 KiB blocks in accumulator
+// order; it adds the chunks in chunk order 0, 1, ..., ksplit-1 (the same
+// order whatever order the chunks ran in) and writes the tile's rows.
+template <int WG>
+__device__ __forceinline__ void sb_mfma_fixup_body(const SbTile *fix_tiles, int nfix, const SbMatBufs *mats,
+                                                   const float *part0, float *y0, int N)
 {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) *fixflag = atomicAdd(&mb.fix[tile.fix], 1) == tile.ksplit - 1;
-    __syncthreads();
-    const bool last = *fixflag != 0;
-    __syncthreads();                               // fixflag is read before the next tile reuses it
-    if (!last) return;
-    __threadfence();
+    const int t = blockIdx.x;
+    if (t >= nfix) return;
+    const SbTile tile = fix_tiles[t];
+    const float *part = mats ? mats[tile.mat].part : part0;
+    float *Y = mats ? mats[tile.mat].y : y0;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int nblk = mt * nt;
-    const sb_f32x4 *__restrict__ part = reinterpret_cast<const sb_f32x4 *>(mb.part) + (int64_t)tile.part_blk * 64;
-    for (int b = wave; b < nblk; b += NW) {
-        sb_f32x4 s = part[(int64_t)b * 64 + lane];
-        for (int c = 1; c < tile.ksplit; ++c) s += part[((int64_t)c * nblk + b) * 64 + lane];
+    const int nt = N >> 4;
+    const int nblk = ((tile.nrows + 15) >> 4) * nt;
+    const sb_f32x4 *__restrict__ P = reinterpret_cast<const sb_f32x4 *>(part) + (int64_t)tile.part_blk * 64;
+    for (int b = wave; b < nblk; b += WG / 64) {
+        sb_f32x4 s = P[(int64_t)b * 64 + lane];
+        for (int c = 1; c < tile.ksplit; ++c) s += P[((int64_t)c * nblk + b) * 64 + lane];
         const int m = b / nt, j = b - m * nt;
         const int rbase = m * 16 + 4 * (lane >> 4);
         const int col = j * 16 + (lane & 15);
@@ -201,24 +218,204 @@ __device__ __forceinline__ void sb_mfma_fixup(const SbTile &tile, const SbMatBuf
         for (int r = 0; r < 4; ++r)
             if (rbase + r < tile.nrows) Y[(size_t)(tile.row0 + rbase + r) * N + col] = s[r];
     }
-    if (threadIdx.x == 0) atomicExch(&mb.fix[tile.fix], 0);
 }
 
-// One MFMA tile (only MFMA pieces; it writes its Y rows).  Work units are
-// (16-row tile mi, column group gi) pairs: ng column groups of nt/ng
-// 16-column tiles, ng the smallest power of two dividing nt that gives every
-// wavefront a unit when the tile is short (or the largest such).  Wavefront w takes units w, w + NW, ...,
-// UNITS of them per pass (one pass for tiles the host cut to
-// SB_MFMA_MAX_ROWS; more only if a generic kernel meets a taller tile).  Per
-// 32-column k-step the workgroup stages the X rows once (split into bf16
-// terms in one of two LDS buffers) and every unit multiplies its A fragment
-// against them.  The X values of k-step s+1 are split into the other buffer
-// after k-step s's MFMAs are issued (the two overlap), their global loads
-// were issued a k-step earlier, and one barrier per k-step orders the
-// buffers.
+// The k-loop of one MFMA piece for a wavefront holding NU units (16-row
+// tiles umi[0..NU) across all N/16 column tiles; NU = 0: the wavefront only
+// helps stage X).  Every wavefront runs the same k-steps and barriers
+// whatever its NU, so the workgroup's waves may run different instances.
+// A k-step is one basic block: the MFMAs of k-step ks (A terms `at`, B
+// fragments read from LDS one column pair ahead), then -- independent of
+// them, so their issue can interleave -- the reload of the A slot, the
+// split of k-step ks+1's A values into the next `at`, the staging of X for
+// k-step ks+1 into the free LDS buffer and the X loads of k-step ks+1+D; one
+// barrier ends the step.  A is loaded D k-steps ahead, X DX k-steps ahead of
+// its staging (SB_MFMA_DEPTH, SB_MFMA_XDEPTH).  Every load is unconditional (k-steps past the piece clamp to
+// its last one), so the compiler's waits stay exact.
+template <int WG, int NRHS, int SPLIT, int NU>
+__device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBufs &mb, const SbPiece &pc,
+                                              const int *umi, int N, int ntw, const int *xrow, uint16_t *B,
+                                              sb_f32x4 (*acc)[SbMfmaCfg<WG, NRHS, SPLIT>::NTMAX])
+{
+    using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
+    constexpr int D = SB_MFMA_DEPTH;
+    constexpr int DX = SB_MFMA_XDEPTH;
+    static_assert(D % DX == 0, "the X depth divides the A depth");
+    constexpr int NUA = NU > 0 ? NU : 1;
+    constexpr int JB = Cfg::NTMAX >= 2 ? 2 : 1;
+    constexpr int NP = (Cfg::NTMAX + JB - 1) / JB;             // column pairs
+    const int lane = threadIdx.x & 63;
+    const int nb = lane & 15, kb = (lane >> 4) * 8;
+    const float *__restrict__ X = mb.x;
+    const int K = pc.ksteps;
+    const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(mb.dval + pc.data_off);
+    float xf[DX][Cfg::UPT][8];
+    sb_f4 ar[D][NUA][2];
+    sb_bf16x8 at[NUA][SPLIT];
+    auto load_a = [&](sb_f4 (*dst)[2], int k) {
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {
+            const int off = (umi[i] * K + k) * 128 + lane;
+            dst[i][0] = __builtin_nontemporal_load(&A[off]);
+            dst[i][1] = __builtin_nontemporal_load(&A[off + 64]);
+        }
+    };
+    auto split_a = [&](sb_f4 (*src)[2]) {
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {
+            const float af[8] = {src[i][0].x, src[i][0].y, src[i][0].z, src[i][0].w,
+                                 src[i][1].x, src[i][1].y, src[i][1].z, src[i][1].w};
+            sb_split8<SPLIT>(af, at[i]);
+        }
+    };
+    // B fragments of column pair jp (a second column past ntw repeats the first)
+    auto read_pair = [&](const uint16_t *Bc, int jp, sb_bf16x8 (*bt)[SPLIT]) {
+#pragma unroll
+        for (int c = 0; c < JB; ++c) {
+            const int j = jp * JB + c;
+            const int n = ((NRHS || j < ntw) && j < Cfg::NTMAX ? j : jp * JB) * 16 + nb;
+#pragma unroll
+            for (int s = 0; s < SPLIT; ++s)
+                bt[c][s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE + n * SB_KP + kb);
+        }
+    };
+#pragma unroll
+    for (int sl = 0; sl < D; ++sl) load_a(ar[sl], sl < K ? sl : K - 1);
+    sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf[0]);
+    sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, 0, xf[0], B, Cfg::TERM_STRIDE);
+#pragma unroll
+    for (int j = 1; j <= DX; ++j)
+        sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, j < K ? j : K - 1, xf[j % DX]);
+    split_a(ar[0]);
+    load_a(ar[0], D < K ? D : K - 1);
+    __syncthreads();
+    // k-step ks: `at` holds its A terms; A slot sl (= ks % D) holds the raw
+    // A of k-step ks + D, slot (sl + 1) % D that of k-step ks + 1
+    auto step = [&](auto slc, const int ks) {
+        constexpr int sl = decltype(slc)::value;
+        const uint16_t *Bc = B + (ks & 1) * Cfg::BUF_STRIDE;
+        if constexpr (NU > 0) {
+            using Pt = SbPasses<SPLIT>;
+            sb_bf16x8 bt[2][JB][SPLIT];
+            read_pair(Bc, 0, bt[0]);
+#pragma unroll
+            for (int jp = 0; jp < NP; ++jp) {
+                if (jp + 1 < NP) read_pair(Bc, jp + 1, bt[(jp + 1) & 1]);
+                if (NRHS || jp * JB < ntw) {
+                    // per pass, the units and the pair's columns: consecutive
+                    // MFMAs go to different accumulators
+#pragma unroll
+                    for (int p = 0; p < Pt::N; ++p)
+#pragma unroll
+                        for (int i = 0; i < NU; ++i)
+#pragma unroll
+                            for (int c = 0; c < JB; ++c)
+                                if (jp * JB + c < Cfg::NTMAX)
+                                    acc[i][jp * JB + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        at[i][Pt::a(p)], bt[jp & 1][c][Pt::b(p)], acc[i][jp * JB + c], 0, 0, 0);
+                }
+            }
+            // the next k-step's A terms, then the slot refilled D ahead
+            constexpr int nsl = (sl + 1) % D;
+            split_a(ar[nsl]);
+            load_a(ar[nsl], ks + 1 + D < K ? ks + 1 + D : K - 1);
+#if defined(SB_MFMA_IGLP) && SB_MFMA_IGLP >= 0
+            __builtin_amdgcn_iglp_opt(SB_MFMA_IGLP);
+#endif
+        }
+        // stage k-step ks + 1 into the free buffer (after the last k-step
+        // this writes zeros nobody reads), then load the X values of
+        // k-step ks + 1 + DX into the slot it frees
+        constexpr int xs = (sl + 1) % DX;
+        sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, ks + 1, xf[xs], B + ((ks + 1) & 1) * Cfg::BUF_STRIDE,
+                                              Cfg::TERM_STRIDE);
+        sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1 + DX < K ? ks + 1 + DX : K - 1, xf[xs]);
+#if SB_MFMA_INTERLEAVE
+        // issue order of the step: the first column pair's B reads, then
+        // every MFMA followed by two VALU (the splits), a B read every
+        // fourth MFMA
+        if constexpr (NU > 0 && NRHS != 0) {
+            constexpr int NM = NU * Cfg::NTMAX * SbPasses<SPLIT>::N;
+            __builtin_amdgcn_sched_group_barrier(0x100, JB * SPLIT, 0);
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+                if (i % 4 == 3) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            }
+        }
+#endif
+        __syncthreads();
+    };
+    int ks = 0;
+    for (; ks + D <= K; ks += D) {
+        step(SbInt<0>{}, ks);
+        if constexpr (D > 1) step(SbInt<(D > 1 ? 1 : 0)>{}, ks + 1);
+        if constexpr (D > 2) step(SbInt<(D > 2 ? 2 : 0)>{}, ks + 2);
+        if constexpr (D > 3) step(SbInt<(D > 3 ? 3 : 0)>{}, ks + 3);
+    }
+    if constexpr (D > 1) if (ks < K) step(SbInt<0>{}, ks);
+    if constexpr (D > 2) if (ks + 1 < K) step(SbInt<(D > 2 ? 1 : 0)>{}, ks + 1);
+    if constexpr (D > 3) if (ks + 2 < K) step(SbInt<(D > 3 ? 2 : 0)>{}, ks + 2);
+}
+
+// One pass of a wavefront holding NU units umi[0..NU) over every piece of
+// the work item, with its accumulators, then its D fragments stored: column
+// tile j of unit u, lane l holds rows 16u + 4(l/16) + r, column 16j + l%16;
+// a chunk of a split tile stores the fragment as its partial-sum block
+// (unit, column tile) in that same lane order instead.
+template <int WG, int NRHS, int SPLIT, int NU>
+__device__ __forceinline__ void sb_mfma_pass(const SbSpmmArgs &a, const SbMatBufs &mb, const SbTile &tile,
+                                             const int *umi, int N, int nt, int mt, int *xrow, uint16_t *B)
+{
+    using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
+    constexpr int NUA = NU > 0 ? NU : 1;
+    const int lane = threadIdx.x & 63;
+    sb_f32x4 acc[NUA][Cfg::NTMAX];
+#pragma unroll
+    for (int u = 0; u < NUA; ++u)
+#pragma unroll
+        for (int j = 0; j < Cfg::NTMAX; ++j) acc[u][j] = sb_f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
+        const SbPiece pc = a.pieces[pi];
+        if (pc.kind != SB_PIECE_MFMA) continue;          // never built into an MFMA tile
+        __syncthreads();                               // previous piece done with xrow and B
+        sb_expand_xrow_mfma(a.xr, pc.xr_begin, pc.xr_end, pc.ncols, xrow);
+        __syncthreads();
+        sb_mfma_kloop<WG, NRHS, SPLIT, NU>(a, mb, pc, umi, N, nt, xrow, B, acc);
+    }
+    const bool split = tile.ksplit > 1;
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const int rbase = umi[i] * 16 + 4 * (lane >> 4);
+#pragma unroll
+        for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
+            if (NRHS || jj < nt) {
+                if (split) {
+                    const int64_t blk = (int64_t)tile.part_blk + (int64_t)(tile.kpart * mt + umi[i]) * nt + jj;
+                    reinterpret_cast<sb_f32x4 *>(mb.part)[blk * 64 + lane] = acc[i][jj];
+                } else {
+                    const int col = jj * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (rbase + r < tile.nrows) mb.y[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
+                }
+            }
+        }
+    }
+}
+
+// One MFMA work item (only MFMA pieces): a row tile, or one k chunk of a
+// split row tile.  Its 16-row units go round robin to the wavefronts, UNITS
+// per wavefront and pass (one pass for tiles the host cut to
+// SB_MFMA_MAX_ROWS; more only if a generic kernel meets a taller tile); each
+// unit covers all N/16 column tiles.  Per 32-column k-step the workgroup
+// stages the X rows once (split into bf16 terms in one of two LDS buffers)
+// and every unit multiplies its A fragment against them (sb_mfma_pass in
+// the instance for the wavefront's unit count; every instance runs the same
+// barriers).
 template <int WG, int NRHS, int SPLIT>
-__device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem,
-                                                  int *fixflag)
+__device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, int *xrow, unsigned char *smem)
 {
     using Cfg = SbMfmaCfg<WG, NRHS, SPLIT>;
     constexpr int NW = Cfg::NW;
@@ -226,185 +423,33 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     const int N = NRHS ? NRHS : a.n_rhs;
     const int nt = N >> 4;
     const SbTile tile = a.tiles[t];
-    const int hs = tile.nrows;
-    const int mt = (hs + 15) >> 4;
+    const int mt = (tile.nrows + 15) >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    // ng divides nt (every column tile is in exactly one group) and NW
-    // (both powers of two, ng <= NW), so the units of one wavefront
-    // (w, w + NW, ...) share the column group w % ng
-    int ng = 1;
-    while (ng * 2 <= nt && nt % (ng * 2) == 0 && mt * ng < NW) ng <<= 1;
-    const int gj0 = (wave % ng) * (nt / ng);
-    const int ntw = nt / ng;
-    const int units = mt * ng;
-    const int mine = units > wave ? (units - wave + NW - 1) / NW : 0;   // units of this wavefront
-    const int passes = (mine + UN - 1) / UN;
+    const int mine = mt > wave ? (mt - wave + NW - 1) / NW : 0;   // units of this wavefront
     const SbMatBufs mb = sb_mat_bufs(a, tile.mat, nullptr);
-    const float *__restrict__ X = mb.x;
-    float *__restrict__ Y = mb.y;
     uint16_t *B = reinterpret_cast<uint16_t *>(smem);
-    const int nb = lane & 15, kb = (lane >> 4) * 8;
-    const bool split = tile.ksplit > 1;
-
     // every wavefront runs the same number of passes (the staging barriers)
-    int all_passes = (((units + NW - 1) / NW) + UN - 1) / UN;
+    int all_passes = (((mt + NW - 1) / NW) + UN - 1) / UN;
     if (all_passes < 1) all_passes = 1;
     for (int pass = 0; pass < all_passes; ++pass) {
-        sb_f32x4 acc[UN][Cfg::NTMAX];
-#pragma unroll
-        for (int u = 0; u < UN; ++u)
-#pragma unroll
-            for (int j = 0; j < Cfg::NTMAX; ++j) acc[u][j] = sb_f32x4{0.f, 0.f, 0.f, 0.f};
-        // this pass's units: index i -> unit wave + NW * (pass * UN + i)
+        // this pass's units: index i -> unit wave + NW * (pass * UN + i);
+        // the active ones are a prefix of nact
         int umi[UN];
-        bool uact[UN];
+        int nact = 0;
 #pragma unroll
         for (int i = 0; i < UN; ++i) {
             const int k = pass * UN + i;
-            const int u = wave + NW * k;
-            uact[i] = k < mine && pass < passes;
-            umi[i] = uact[i] ? u / ng : 0;
+            const bool act = k < mine;
+            umi[i] = act ? wave + NW * k : 0;
+            nact += act;
         }
-        for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
-            const SbPiece pc = a.pieces[pi];
-            if (pc.kind != SB_PIECE_MFMA) continue;          // never built into an MFMA tile
-            __syncthreads();                               // previous piece done with xrow and B
-            sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
-            __syncthreads();
-            const int K = pc.ksteps;
-            const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(mb.dval + pc.data_off);
-            // pipeline, DEPTH k-steps deep: A of k-step s sits in register
-            // slot s % DEPTH and X of k-step s in slot s % DEPTH, each loaded
-            // DEPTH k-steps before its use (A is streamed from HBM, so one
-            // k-step of compute cannot cover its latency).  B of k-step s is
-            // in LDS buffer s & 1; X of k-step s + 1 is split into the free
-            // buffer after k-step s's MFMAs are issued.  Every load is
-            // unconditional (indices past the piece clamp to its last k-step)
-            // so the waits on them stay exact; one barrier per k-step.
-            constexpr int D = SB_MFMA_DEPTH;
-            float xf[D][Cfg::UPT][8];
-            sb_f4 ar[D][UN][2];
-#pragma unroll
-            for (int sl = 0; sl < D; ++sl) {
-                const int kl = sl < K ? sl : K - 1;
-#pragma unroll
-                for (int i = 0; i < UN; ++i) {
-                    const int off = (umi[i] * K + kl) * 128 + lane;
-                    ar[sl][i][0] = __builtin_nontemporal_load(&A[off]);
-                    ar[sl][i][1] = __builtin_nontemporal_load(&A[off + 64]);
-                }
-            }
-            sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf[0]);
-            sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, 0, xf[0], B, Cfg::TERM_STRIDE);
-#pragma unroll
-            for (int j = 1; j <= D; ++j)
-                sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, j < K ? j : K - 1, xf[j % D]);
-            __syncthreads();
-            // one k-step with A in register slot sl and the X values of k-step
-            // ks + 1 in slot (sl + 1) % D
-            auto step = [&](auto slc, const int ks) {
-                constexpr int sl = decltype(slc)::value;
-                const uint16_t *Bc = B + (ks & 1) * Cfg::BUF_STRIDE;
-                // A terms of every unit (the slot is then reloaded with
-                // k-step ks + D), then each B fragment read once from LDS
-                // and applied to all units (units of a wavefront share
-                // their column group)
-                sb_bf16x8 at[UN][SPLIT];
-#pragma unroll
-                for (int i = 0; i < UN; ++i) {
-                    const float af[8] = {ar[sl][i][0].x, ar[sl][i][0].y, ar[sl][i][0].z, ar[sl][i][0].w,
-                                         ar[sl][i][1].x, ar[sl][i][1].y, ar[sl][i][1].z, ar[sl][i][1].w};
-                    sb_split8<SPLIT>(af, at[i]);
-                }
-                {
-                    const int kn = ks + D < K ? ks + D : K - 1;
-#pragma unroll
-                    for (int i = 0; i < UN; ++i) {
-                        const int off = (umi[i] * K + kn) * 128 + lane;
-                        ar[sl][i][0] = __builtin_nontemporal_load(&A[off]);
-                        ar[sl][i][1] = __builtin_nontemporal_load(&A[off + 64]);
-                    }
-                }
-                // two column tiles per step: per unit, its 2 x passes MFMAs
-                // alternate between the two accumulators, so no MFMA waits on
-                // the one before it.  A second column past ntw reads column
-                // j0's B and is never stored.  A unit's branch is uniform
-                // over the wavefront (inactive units issue nothing).
-                constexpr int JB = Cfg::NTMAX >= 2 ? 2 : 1;
-#pragma unroll
-                for (int j0 = 0; j0 < Cfg::NTMAX; j0 += JB) {
-                    if (j0 < ntw) {
-                        sb_bf16x8 bt[JB][SPLIT];
-#pragma unroll
-                        for (int c = 0; c < JB; ++c) {
-                            const int n = (gj0 + (j0 + c < ntw ? j0 + c : j0)) * 16 + nb;
-#pragma unroll
-                            for (int s = 0; s < SPLIT; ++s)
-                                bt[c][s] = *reinterpret_cast<const sb_bf16x8 *>(Bc + s * Cfg::TERM_STRIDE +
-                                                                             n * SB_KP + kb);
-                        }
-                        using Pt = SbPasses<SPLIT>;
-#pragma unroll
-                        for (int i = 0; i < UN; ++i) {
-                            if (!uact[i]) continue;
-#pragma unroll
-                            for (int p = 0; p < Pt::N; ++p)
-#pragma unroll
-                                for (int c = 0; c < JB; ++c)
-                                    if (j0 + c < Cfg::NTMAX)   // odd NTMAX: the last pair has one column
-                                        acc[i][j0 + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                            at[i][Pt::a(p)], bt[c][Pt::b(p)], acc[i][j0 + c], 0, 0, 0);
-                        }
-                    }
-                }
-                // stage k-step ks + 1 into the free buffer (after the last
-                // k-step this writes zeros nobody reads), then load the X
-                // values of k-step ks + 1 + D into the slot it frees
-                constexpr int xs = (sl + 1) % D;
-                sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, ks + 1, xf[xs],
-                                                      B + ((ks + 1) & 1) * Cfg::BUF_STRIDE, Cfg::TERM_STRIDE);
-                sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1 + D < K ? ks + 1 + D : K - 1, xf[xs]);
-                __syncthreads();
-            };
-            // groups of D k-steps in a loop with one back-edge and no other
-            // branch around a load, then the remaining k-steps
-            int ks = 0;
-            for (; ks + D <= K; ks += D) {
-                step(SbInt<0>{}, ks);
-                if constexpr (D > 1) step(SbInt<(D > 1 ? 1 : 0)>{}, ks + 1);
-                if constexpr (D > 2) step(SbInt<(D > 2 ? 2 : 0)>{}, ks + 2);
-                if constexpr (D > 3) step(SbInt<(D > 3 ? 3 : 0)>{}, ks + 3);
-            }
-            if constexpr (D > 1) if (ks < K) step(SbInt<0>{}, ks);
-            if constexpr (D > 2) if (ks + 1 < K) step(SbInt<(D > 2 ? 1 : 0)>{}, ks + 1);
-            if constexpr (D > 3) if (ks + 2 < K) step(SbInt<(D > 3 ? 2 : 0)>{}, ks + 2);
-        }
-        // D fragment of column tile j: lane holds rows 16*mi + 4*(lane/16) + r,
-        // column 16*j + lane%16.  A chunk of a split tile stores it as its
-        // partial-sum block (unit, column tile) in that same order instead.
-#pragma unroll
-        for (int i = 0; i < UN; ++i) {
-            if (!uact[i]) continue;
-            const int j0 = gj0;
-            const int rbase = umi[i] * 16 + 4 * (lane >> 4);
-#pragma unroll
-            for (int jj = 0; jj < Cfg::NTMAX; ++jj) {
-                if (jj < ntw) {
-                    if (split) {
-                        const int64_t blk = (int64_t)tile.part_blk + (int64_t)(tile.kpart * mt + umi[i]) * nt + j0 + jj;
-                        reinterpret_cast<sb_f32x4 *>(mb.part)[blk * 64 + lane] = acc[i][jj];
-                    } else {
-                        const int col = (j0 + jj) * 16 + nb;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (rbase + r < hs) Y[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
-                    }
-                }
-            }
-        }
+        if (nact >= UN)
+            sb_mfma_pass<WG, NRHS, SPLIT, UN>(a, mb, tile, umi, N, nt, mt, xrow, B);
+        else if (UN > 1 && nact == 1)
+            sb_mfma_pass<WG, NRHS, SPLIT, 1>(a, mb, tile, umi, N, nt, mt, xrow, B);
+        else
+            sb_mfma_pass<WG, NRHS, SPLIT, 0>(a, mb, tile, umi, N, nt, mt, xrow, B);
     }
-    if (split) sb_mfma_fixup<NW>(tile, mb, Y, N, nt, mt, fixflag);
 }
 
 template <int WG, int NRHS, int XCAP, int SPLIT>
@@ -414,7 +459,6 @@ __device__ __forceinline__ void sb_spmm_mfma_body(const SbSpmmArgs &a)
     __shared__ __attribute__((aligned(16))) int xrow[XCAP + 32];   // k-steps read up to 31 past ncols
     __shared__ __attribute__((aligned(16))) unsigned char smem[Cfg::SMEM_BYTES];
     __shared__ int next_tile;
-    __shared__ int fixflag;
     sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmm_mfma_tile<WG, NRHS, SPLIT>(a, t, xrow, smem, &fixflag); });
+                 [&](int t) { sb_spmm_mfma_tile<WG, NRHS, SPLIT>(a, t, xrow, smem); });
 }

@@ -6,9 +6,10 @@ exactly max(1, round(d*r*c)) nonzeros at uniform positions, N uniform in
 {1, 8, 64, 256}.
 
 Labels are per-block costs measured by the kernels themselves.  Blocks are
-grouped by (log2 rows, log2 cols, density bin, N); for every group and every
-route that applies (dense FP32, sparse, and the MFMA route when N is a
-multiple of 16) one block-diagonal matrix holding the group's blocks
+grouped by (log2 rows, log2 cols, density bin, N); for every group, every
+tile shape (the launch's workgroup size, SHAPES) and every route that
+applies (dense FP32, sparse, and the MFMA route when N is a multiple of 16)
+one block-diagonal matrix holding the group's blocks
 (replicated until the launch streams at least ``min_bytes``) is timed with
 HIP events, starting cold (a 300 MB buffer larger than the last-level cache
 is overwritten before each launch).  The per-block cost is (median launch
@@ -27,6 +28,14 @@ from .features import block_features, encode
 N_CHOICES = (1, 8, 64, 256)
 DENSITY_EDGES = np.array([0.005, 0.0125, 0.03, 0.08, 0.2, 0.5, 1.0000001])
 ROUTES = (convert.ROUTE_DENSE, convert.ROUTE_SPARSE, convert.ROUTE_DENSE_MFMA)
+# tile shapes the router chooses between: the workgroup size of the launch
+# (narrow, wide), per operation.  Narrow workgroups suit short tiles (small
+# blocks, small matrices); wide ones long streams.
+SHAPES = {"spmv": (256, 512), "spmm": (128, 256)}
+
+
+def shape_wg(op, s):
+    return SHAPES["spmv" if op == "spmv" else "spmm"][s]
 
 
 def sample_blocks(n_blocks, seed):
@@ -73,24 +82,27 @@ def block_diagonal(rs, cs, ds, rng):
 
 
 def measure_groups(r, c, d, N, seed=0, min_bytes=6 << 20, max_blocks=4096, reps=5, progress=None):
-    """Times every (group, route).  Returns (keys, costs_ns [G, 3] with NaN
-    where a route does not apply, group index of every block)."""
+    """Times every (group, shape, route).  Returns (keys, costs_ns [G, 6]
+    (column shape * 3 + route) with NaN where a route does not apply, group
+    index of every block)."""
     from ..runtime import Plan, DeviceArray
     keys = group_keys(r, c, d, N)
     uniq, inv = np.unique(keys, axis=0, return_inverse=True)
     inv = inv.ravel()
     rng = np.random.default_rng(seed)
-    costs = np.full((len(uniq), 3), np.nan)
-    # empty-launch overhead per op: one tiny block
+    ns = len(SHAPES["spmv"])
+    costs = np.full((len(uniq), ns * len(ROUTES)), np.nan)
+    # empty-launch overhead per op and shape: one tiny block
     base = {}
     tiny = block_diagonal([4], [4], [1.0], rng)
     for n in N_CHOICES:
         op = "spmv" if n == 1 else "spmm"
-        p = Plan.from_vbr(tiny, [0], op=op, n_rhs=n)
         xd = DeviceArray.from_numpy(np.ones((4,) if n == 1 else (4, n), np.float32))
         yd = DeviceArray((4,) if n == 1 else (4, n))
-        base[n] = float(np.median(p.time(xd, yd, warmup=3, reps=reps, flush_bytes=300 << 20)))
-        p.close()
+        for s in range(ns):
+            p = Plan.from_vbr(tiny, [0], op=op, n_rhs=n, wg=shape_wg(op, s))
+            base[n, s] = float(np.median(p.time(xd, yd, warmup=3, reps=reps, flush_bytes=300 << 20)))
+            p.close()
     for g in range(len(uniq)):
         members = np.nonzero(inv == g)[0]
         n = int(uniq[g, 3])
@@ -105,13 +117,14 @@ def measure_groups(r, c, d, N, seed=0, min_bytes=6 << 20, max_blocks=4096, reps=
         X = rng.uniform(-1, 1, (ncols,) if n == 1 else (ncols, n)).astype(np.float32)
         xd = DeviceArray.from_numpy(X)
         yd = DeviceArray((nrows,) if n == 1 else (nrows, n))
-        for ri, route in enumerate(ROUTES):
-            if route == convert.ROUTE_DENSE_MFMA and (n == 1 or n % 16):
-                continue
-            plan = Plan.from_vbr(vbr, np.full(count, route, np.uint8), op=op, n_rhs=n)
-            t = float(np.median(plan.time(xd, yd, warmup=2, reps=reps, flush_bytes=300 << 20)))
-            plan.close()
-            costs[g, ri] = max(t - base[n], 1e-6) * 1e6 / count     # ns per block
+        for s in range(ns):
+            for ri, route in enumerate(ROUTES):
+                if route == convert.ROUTE_DENSE_MFMA and (n == 1 or n % 16):
+                    continue
+                plan = Plan.from_vbr(vbr, np.full(count, route, np.uint8), op=op, n_rhs=n, wg=shape_wg(op, s))
+                t = float(np.median(plan.time(xd, yd, warmup=2, reps=reps, flush_bytes=300 << 20)))
+                plan.close()
+                costs[g, s * len(ROUTES) + ri] = max(t - base[n, s], 1e-6) * 1e6 / count     # ns per block
         xd.free()
         yd.free()
         if progress and (g % 50 == 0 or g == len(uniq) - 1):

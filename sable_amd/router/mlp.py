@@ -10,13 +10,23 @@ from ..runtime import DeviceArray, init
 N_FEATURES = 6
 N_HIDDEN = 32
 N_ROUTES = 3          # dense FP32, sparse, dense MFMA
+N_SHAPES = 2          # workgroup shapes (see labels.SHAPES)
+N_OUT = N_ROUTES * N_SHAPES   # output k = shape * N_ROUTES + route
 
 
-def n_params(F=N_FEATURES, H=N_HIDDEN, K=N_ROUTES):
+def n_params(F=N_FEATURES, H=N_HIDDEN, K=N_OUT):
     return H * F + H + K * H + K
 
 
-def init_params(seed=0, F=N_FEATURES, H=N_HIDDEN, K=N_ROUTES):
+def outputs_of(params, F=N_FEATURES, H=N_HIDDEN):
+    """The output count K of a parameter vector of a 2-layer F-H-K MLP."""
+    k, r = divmod(np.asarray(params).size - H * F - H, H + 1)
+    if r or k <= 0:
+        raise ValueError("parameter vector does not match an MLP with these F, H")
+    return int(k)
+
+
+def init_params(seed=0, F=N_FEATURES, H=N_HIDDEN, K=N_OUT):
     rng = np.random.default_rng(seed)
     W1 = rng.normal(0, np.sqrt(2.0 / F), (H, F))
     b1 = np.zeros(H)
@@ -28,8 +38,9 @@ def init_params(seed=0, F=N_FEATURES, H=N_HIDDEN, K=N_ROUTES):
 class DeviceMLP:
     """Parameters and optimizer state on the device."""
 
-    def __init__(self, params, F=N_FEATURES, H=N_HIDDEN, K=N_ROUTES):
+    def __init__(self, params, F=N_FEATURES, H=N_HIDDEN, K=None):
         init()
+        K = outputs_of(params, F, H) if K is None else K
         self.F, self.H, self.K = F, H, K
         p = np.ascontiguousarray(params, dtype=np.float32)
         assert p.size == n_params(F, H, K)

@@ -42,14 +42,17 @@ def allowed_mask(n_rhs, nblocks):
 
 
 class Router:
-    """A trained MLP plus the feature standardisation it was trained with."""
+    """A trained MLP plus the feature standardisation it was trained with.
+    The network predicts, per block, the log2 ns cost of every route under
+    every tile shape (output shape * 3 + route; labels.SHAPES)."""
 
     def __init__(self, params, mean, std, meta=""):
-        from .mlp import DeviceMLP
+        from .mlp import DeviceMLP, N_ROUTES
         self.mlp = DeviceMLP(params)
         self.mean = np.asarray(mean, dtype=np.float64)
         self.std = np.asarray(std, dtype=np.float64)
         self.meta = meta
+        self.n_shapes = max(1, self.mlp.K // N_ROUTES)
 
     @classmethod
     def load(cls, path=DEFAULT_WEIGHTS):
@@ -62,23 +65,47 @@ class Router:
         return os.path.exists(path)
 
     def predict_costs(self, F):
-        """log2 ns per block for each route, [n, 3]."""
+        """log2 ns per block, [n, shapes * 3] (column shape * 3 + route)."""
         E = (encode(F) - self.mean) / self.std
         return self.mlp.predict(E.astype(np.float32))
 
-    def route_blocks(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,
-                     mfma_bias=0.0):
-        """The cheapest allowed route per block; ``sparse_bias`` and
-        ``mfma_bias`` (log2 ns) are subtracted from the predicted costs of the
-        sparse and MFMA routes first."""
+    def _costs(self, F, n_rhs, sparse_bias, mfma_bias):
+        pred = self.predict_costs(F).astype(np.float64).reshape(len(F), self.n_shapes, 3)
+        pred[:, :, 1] -= float(sparse_bias)
+        pred[:, :, 2] -= float(mfma_bias)
+        pred[np.broadcast_to(~allowed_mask(n_rhs, len(F))[:, None, :], pred.shape)] = np.inf
+        return pred
+
+    def route_and_shape(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,
+                        mfma_bias=0.0, shape=None):
+        """The tile shape and the route of every block: per shape, each block
+        takes its cheapest allowed route (``sparse_bias`` / ``mfma_bias``,
+        log2 ns, are subtracted from the predicted costs of the sparse and
+        MFMA routes first); the plan takes the shape whose blocks' predicted
+        costs (2^log2) add up to the least, unless ``shape`` fixes it.
+        Returns (routes, shape index, workgroup size)."""
+        from .labels import shape_wg
         F = block_features(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs)
+        op = "spmv" if n_rhs == 1 else "spmm"
         if len(F) == 0:
-            return np.zeros(0, dtype=np.uint8)
-        pred = self.predict_costs(F).astype(np.float64)
-        pred[:, 1] -= float(sparse_bias)
-        pred[:, 2] -= float(mfma_bias)
-        pred[~allowed_mask(n_rhs, len(F))] = np.inf
-        return np.asarray(ROUTES, dtype=np.uint8)[np.argmin(pred, axis=1)]
+            s = 0 if shape is None else int(shape)
+            return np.zeros(0, dtype=np.uint8), s, shape_wg(op, s)
+        pred = self._costs(F, n_rhs, sparse_bias, mfma_bias)
+        best = np.argmin(pred, axis=2)                            # [n, shapes]
+        if shape is None:
+            tot = np.sum(np.exp2(np.take_along_axis(pred, best[:, :, None], axis=2)[:, :, 0]), axis=0)
+            s = int(np.argmin(tot))
+        else:
+            s = int(shape)
+        routes = np.asarray(ROUTES, dtype=np.uint8)[best[:, s]]
+        return routes, s, shape_wg(op, s)
+
+    def route_blocks(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,
+                     mfma_bias=0.0, shape=None):
+        """The cheapest allowed route per block (under the chosen shape, see
+        route_and_shape)."""
+        return self.route_and_shape(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs, sparse_bias,
+                                    mfma_bias, shape)[0]
 
 
 def fixed_routes(vbr, n_rhs=1, threshold=convert.DEFAULT_DENSITY, mfma=False):
@@ -103,8 +130,11 @@ def choose_plan(vbr, x, router=None, n_rhs=1, threshold=convert.DEFAULT_DENSITY,
     shape_y = (int(vbr.rpntr[-1]),) if n_rhs == 1 else (int(vbr.rpntr[-1]), n_rhs)
     yd = DeviceArray(shape_y)
 
-    def run(routes):
-        p = Plan.from_vbr(vbr, routes, op=op, n_rhs=n_rhs, **plan_kw)
+    def run(routes, wg=None):
+        kw = dict(plan_kw)
+        if wg is not None:
+            kw["wg"] = wg
+        p = Plan.from_vbr(vbr, routes, op=op, n_rhs=n_rhs, **kw)
         t = float(np.median(p.time(xd, yd, warmup=3, reps=reps, flush_bytes=flush_bytes)))
         y = yd.numpy()
         return p, t, y
@@ -120,26 +150,26 @@ def choose_plan(vbr, x, router=None, n_rhs=1, threshold=convert.DEFAULT_DENSITY,
     seen = []
     report["candidates"] = []
     for sb, mb in (BIAS_GRID[op] if biases is None else biases):
-        learned = router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                      vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
-        if any(np.array_equal(learned, r) for r in seen):
+        learned, _, wg = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                                vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
+        if any(np.array_equal(learned, r) and wg == w for r, w in seen):
             continue
-        seen.append(learned)
-        p, t, y = run(learned)
+        seen.append((learned, wg))
+        p, t, y = run(learned, wg)
         ok = fixed_ok and _within(vbr, x, y, n_rhs, tol)
-        report["candidates"].append({"sparse_bias": sb, "mfma_bias": mb, "ms": t, "within_tol": bool(ok),
+        report["candidates"].append({"sparse_bias": sb, "mfma_bias": mb, "wg": wg, "ms": t, "within_tol": bool(ok),
                                      "routes": {int(k): int(v) for k, v in
                                                 zip(*np.unique(learned, return_counts=True))}})
         if ok and (best is None or t < best[1]):
             if best is not None:
                 best[0].close()
-            best = (p, t, sb, mb, learned)
+            best = (p, t, sb, mb, learned, wg)
         else:
             p.close()
     report["learned_within_tol"] = best is not None
     if best is not None:
         report["learned_ms"] = best[1]
-        report["sparse_bias"], report["mfma_bias"] = best[2], best[3]
+        report["sparse_bias"], report["mfma_bias"], report["wg"] = best[2], best[3], best[5]
         report["learned_routes"] = {int(k): int(v) for k, v in zip(*np.unique(best[4], return_counts=True))}
         if best[1] < t_fixed:
             report["kept"] = "learned"

@@ -74,16 +74,19 @@ def evaluate(router, n_matrices=20, seed=4000, reps=15, progress=log):
         vbr, x, _ = synth.workload_like_c2(n, seed=seed + i)
         xd = DeviceArray.from_numpy(x)
         yd = DeviceArray(int(vbr.rpntr[-1]))
+        learned, shape, wg = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                    vbr.bpntrb, vbr.bpntre, 1)
         plans = {
             "fixed": convert.threshold_routes(vbr.val, vbr.indx, convert.DEFAULT_DENSITY),
             "all_dense": np.zeros(len(vbr.bindx), np.uint8),
             "all_sparse": np.ones(len(vbr.bindx), np.uint8),
-            "learned": router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                           vbr.bpntre, 1),
+            "learned": learned,
         }
-        row = {"n": n, "blocks": int(len(vbr.bindx)), "nnz": int(np.count_nonzero(vbr.val))}
+        row = {"n": n, "blocks": int(len(vbr.bindx)), "nnz": int(np.count_nonzero(vbr.val)), "learned_wg": wg}
         for name, routes in plans.items():
-            p = Plan.from_vbr(vbr, routes)
+            # the learned plan runs in the tile shape the router chose; the
+            # others in the tuned default shape
+            p = Plan.from_vbr(vbr, routes, **({"wg": wg} if name == "learned" else {}))
             t = float(np.median(p.time(xd, yd, warmup=3, reps=reps, flush_bytes=1 << 30)))
             row[name + "_ms"] = t
             row[name + "_ok"] = _within(vbr, x, yd.numpy(), 1, 1e-5)

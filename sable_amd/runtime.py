@@ -220,7 +220,7 @@ class Plan:
                            ("dense_mcols", "<i4"), ("sp_glog", "<i4"),
                            ("sp_map", "<i4"), ("sp_waves", "<i4"),
                            ("y_accum", "<i4"), ("stage_map", "<i4"), ("stage_len", "<i4"),
-                           ("ksplit", "<i4"), ("kpart", "<i4"), ("fix", "<i4"), ("part_blk", "<i4")])
+                           ("ksplit", "<i4"), ("kpart", "<i4"), ("part_blk", "<i4")])
     PIECE_DTYPE = np.dtype([("data_off", "<i8"), ("kind", "<i4"), ("ncols", "<i4"),
                             ("xr_begin", "<i4"), ("xr_end", "<i4"), ("rowptr_off", "<i4"),
                             ("ksteps", "<i4"), ("grp_end", "<i4"), ("stage_len", "<i4"),

@@ -163,7 +163,7 @@ def test_c5_group_of_256_matrices(rt):
         for op, inp in (("spmv", x), ("spmm", X)):
             k = 1 if op == "spmv" else 16
             routes = (router.route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                          vbr.bpntre, k, mfma_bias=1.0 if k > 1 else 0.0) if router else fixed)
+                                          vbr.bpntre, k, mfma_bias=1.0 if k > 1 else 0.0, shape=0) if router else fixed)
             p = rt.Plan.from_vbr(vbr, routes, op=op, n_rhs=k, min_tiles=1, wg=128 if k > 1 else 256)
             shape = (n,) if k == 1 else (n, k)
             xd = rt.DeviceArray.from_numpy(inp)

@@ -28,7 +28,7 @@ def _data(n=300, F=6, K=3, seed=0):
 def test_forward_and_gradient_match_oracle(mods):
     rt, mlp = mods
     F, H, K = 6, 32, 3
-    p = mlp.init_params(1)
+    p = mlp.init_params(1, K=K)
     X, T, M = _data()
     n = len(X)
     net = mlp.DeviceMLP(p)
@@ -44,7 +44,7 @@ def test_forward_and_gradient_match_oracle(mods):
 
 def test_adam_step_matches_oracle(mods):
     rt, mlp = mods
-    p = mlp.init_params(2)
+    p = mlp.init_params(2, K=3)
     X, T, M = _data(seed=3)
     n = len(X)
     net = mlp.DeviceMLP(p)
@@ -63,7 +63,7 @@ def test_training_lowers_loss(mods):
     W = rng.normal(0, 1, (6, 3))
     T = (np.tanh(X @ W) + 0.1 * X[:, :3]).astype(np.float32)
     M = np.ones_like(T)
-    net = mlp.DeviceMLP(mlp.init_params(0))
+    net = mlp.DeviceMLP(mlp.init_params(0, K=3))
     print("case: mlp training", flush=True)
     hist = net.train(X, T, M, steps=300, batch=512, lr=1e-2, log_every=100)
     assert hist[-1][1] < 0.5 * hist[0][1]
@@ -140,9 +140,10 @@ def test_route_biases(mods):
     r = Router.load()
     args = (vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
     print("case: route biases on an 8k matrix", flush=True)
-    base = r.route_blocks(*args, 1)
-    assert np.array_equal(base, r.route_blocks(*args, 1, sparse_bias=0.0, mfma_bias=0.0))
-    counts = [int(np.sum(r.route_blocks(*args, 1, sparse_bias=b) == ROUTE_SPARSE)) for b in (-100, 0, 0.5, 1, 100)]
+    base = r.route_blocks(*args, 1, shape=1)
+    assert np.array_equal(base, r.route_blocks(*args, 1, sparse_bias=0.0, mfma_bias=0.0, shape=1))
+    counts = [int(np.sum(r.route_blocks(*args, 1, sparse_bias=b, shape=1) == ROUTE_SPARSE))
+              for b in (-100, 0, 0.5, 1, 100)]
     assert counts[0] == 0 and counts[-1] == len(base) and counts == sorted(counts)
     assert not np.any(r.route_blocks(*args, 1, mfma_bias=100) == ROUTE_DENSE_MFMA)
     assert np.all(r.route_blocks(*args, 128, mfma_bias=100) == ROUTE_DENSE_MFMA)
