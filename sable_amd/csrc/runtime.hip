@@ -349,7 +349,7 @@ struct SablePlan {
     // MFMA tiles split along k: partial-sum scratch and the fixup list
     float *d_part = nullptr;
     SbTile *d_fix_tiles = nullptr;
-    int32_t nfix = 0;
+    int32_t nfix = 0, fix_groups = 1;
     // host staging for SABLE_HOST_PTRS
     float *d_xtmp = nullptr, *d_ytmp = nullptr;
     int64_t xtmp_n = 0, ytmp_n = 0;
@@ -374,7 +374,7 @@ struct GroupPart {
     SbMatBufs *d_mats = nullptr;
     int32_t *d_queue = nullptr;
     SbTile *d_fix_tiles = nullptr;   // the members' split MFMA row tiles (matrix slot in .mat)
-    int32_t nfix = 0;
+    int32_t nfix = 0, fix_groups = 1;
     int32_t ntiles_mfma = 0;         // MFMA tiles (the first phase when nonzero)
     std::vector<Phase> phases;
 };
@@ -558,13 +558,13 @@ int launch_phase(int op, int n_rhs, const Phase &ph, int wg, int mode, const SbT
     return launch_raw(ph.rtc, ph.fn, ph.aot, grid, wg, &a, st, e0, e1);
 }
 
-// Launches the fixup of a plan's (or group part's) split MFMA tiles: one
-// workgroup per split row tile.
-int launch_fixup(const SbTile *fix_tiles, int nfix, const SbMatBufs *mats, const float *part, float *y, int n_rhs,
-                 hipStream_t st)
+// Launches the fixup of a plan's (or group part's) split MFMA tiles:
+// `groups` workgroups per split row tile.
+int launch_fixup(const SbTile *fix_tiles, int nfix, int groups, const SbMatBufs *mats, const float *part, float *y,
+                 int n_rhs, hipStream_t st)
 {
     if (nfix <= 0) return SABLE_OK;
-    hipLaunchKernelGGL(sb_mfma_fixup, dim3(nfix), dim3(256), 0, st, fix_tiles, nfix, mats, part, y, n_rhs);
+    hipLaunchKernelGGL(sb_mfma_fixup, dim3(nfix, groups), dim3(256), 0, st, fix_tiles, nfix, mats, part, y, n_rhs);
     HIPOK(hipGetLastError());
     return SABLE_OK;
 }
@@ -586,7 +586,7 @@ int plan_launch_with(SablePlan *p, const float *x, float *y, float *part, hipEve
                              p->d_xr, p->d_dval, p->d_sval, p->d_sidx, p->d_srowptr, p->d_xmap, p->d_queue, nullptr,
                              x, y, part, p->stream, attached ? e0 : nullptr, attached ? e1 : nullptr));
         if (i == 0 && p->stats.ntiles_mfma > 0)
-            CHECKRC(launch_fixup(p->d_fix_tiles, p->nfix, nullptr, part, y, p->host.n_rhs, p->stream));
+            CHECKRC(launch_fixup(p->d_fix_tiles, p->nfix, p->fix_groups, nullptr, part, y, p->host.n_rhs, p->stream));
     }
     if (e1 && !attached) HIPOK(hipEventRecord(e1, p->stream));
     return SABLE_OK;
@@ -729,6 +729,7 @@ int sable_plan_create(const SableVbr *vbr, const uint8_t *route, const SablePlan
         HIPOK(hipMalloc((void **)&p->d_part, (size_t)h.part_blocks * 1024));
         CHECKRC(upload(&p->d_fix_tiles, h.fix_tiles, bytes));
         p->nfix = (int32_t)h.fix_tiles.size();
+        p->fix_groups = sb_fixup_groups(h.fix_tiles.data(), h.fix_tiles.size(), h.n_rhs);
         bytes += h.part_blocks * 1024;
     }
     HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
@@ -985,6 +986,7 @@ int build_part(SablePlan *const *plans, const std::vector<int> &idx, const float
     CHECKRC(upload(&q.d_mats, mats, bytes));
     CHECKRC(upload(&q.d_fix_tiles, fixes, bytes));
     q.nfix = (int32_t)fixes.size();
+    q.fix_groups = sb_fixup_groups(fixes.data(), fixes.size(), q.n_rhs);
     HIPOK(hipMalloc((void **)&q.d_queue, 2 * sizeof(int32_t)));
     HIPOK(hipMemset(q.d_queue, 0, 2 * sizeof(int32_t)));
     // kernel: the specialisation machinery over a host plan view of the part
@@ -1006,7 +1008,7 @@ int launch_part(const GroupPart &q, hipStream_t st)
         CHECKRC(launch_phase(q.op, q.n_rhs, q.phases[i], q.wg, q.mode, q.d_tiles, q.d_pieces, q.d_xr, nullptr, nullptr,
                              nullptr, nullptr, nullptr, q.d_queue, q.d_mats, nullptr, nullptr, nullptr, st, nullptr,
                              nullptr));
-        if (i == 0 && has_mfma_phase) CHECKRC(launch_fixup(q.d_fix_tiles, q.nfix, q.d_mats, nullptr, nullptr, q.n_rhs, st));
+        if (i == 0 && has_mfma_phase) CHECKRC(launch_fixup(q.d_fix_tiles, q.nfix, q.fix_groups, q.d_mats, nullptr, nullptr, q.n_rhs, st));
     }
     return SABLE_OK;
 }

@@ -31,6 +31,23 @@ typedef float sb_f4 __attribute__((ext_vector_type(4)));
 typedef unsigned int sb_u4 __attribute__((ext_vector_type(4)));
 typedef unsigned int sb_u2 __attribute__((ext_vector_type(2)));
 
+// Global-address-space views of the buffer pointers.  The pointers come
+// from descriptor tables (SbMatBufs), so the compiler cannot tell they point
+// to global memory and would emit FLAT loads and stores; those count against
+// the LDS counter as well, so every wait for an LDS access would also wait
+// for every global load in flight, defeating the software pipelining.
+#define SB_G __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ SB_G T *sb_gp(T *p)
+{
+    return (SB_G T *)p;
+}
+template <typename T>
+__device__ __forceinline__ const SB_G T *sb_gp(const T *p)
+{
+    return (const SB_G T *)p;
+}
+
 #define SB_MODE_GRID 0      // one workgroup per tile
 #define SB_MODE_STATIC 1    // persistent, tile = wg + k * grid
 #define SB_MODE_QUEUE 2     // persistent, atomic work queue (self-resetting)
@@ -178,7 +195,11 @@ __device__ __forceinline__ void sb_for_tiles(int ntiles, int mode, int32_t *queu
 
 // The waves per SIMD the SpMV kernel is compiled for (its VGPR budget) given
 // the dense unroll: 80 VGPRs (6 waves) at unroll 2, 96 (5 waves) at unroll 4.
-#define SB_SPMV_WAVES(U) ((U) <= 2 ? 6 : (U) <= 4 ? 5 : 1)
+// SB_SPMV_MINWAVES > 0 sets it for every unroll.
+#ifndef SB_SPMV_MINWAVES
+#define SB_SPMV_MINWAVES 0
+#endif
+#define SB_SPMV_WAVES(U) (SB_SPMV_MINWAVES > 0 ? SB_SPMV_MINWAVES : (U) <= 2 ? 6 : (U) <= 4 ? 5 : 1)
 
 struct SbStreamLane {
     int c0, c1, c2, c3;    // column offsets of this lane's four elements
@@ -189,7 +210,7 @@ struct SbStreamLane {
 // is still in flight: an index past the end reads the zero float4 the plan
 // builder appends after every dense piece (P[nq]).
 template <int UNROLL>
-__device__ __forceinline__ void sb_dense_load(const sb_f4 *__restrict__ P, int nq, int q0, int tpar,
+__device__ __forceinline__ void sb_dense_load(const SB_G sb_f4 *__restrict__ P, int nq, int q0, int tpar,
                                               sb_f4 *v)
 {
 #pragma unroll
@@ -217,7 +238,7 @@ __device__ __forceinline__ void sb_dense_fma(const sb_f4 *v, const float *xs, co
 
 // Streams one dense piece whose x slice starts at xs.
 template <int UNROLL>
-__device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int nq, int tpar, int M, int ncols,
+__device__ __forceinline__ void sb_dense_piece(const SB_G sb_f4 *__restrict__ P, int nq, int tpar, int M, int ncols,
                                                const float *xs, const SbStreamLane &L, float &a0, float &a1,
                                                float &a2, float &a3)
 {
@@ -246,8 +267,8 @@ __device__ __forceinline__ void sb_dense_piece(const sb_f4 *__restrict__ P, int 
 // -1 for the zero pads): two dependent loads per slot, 4 slots in flight per
 // thread, no search over descriptors.
 template <int WG>
-__device__ __forceinline__ void sb_stage_map(const int32_t *__restrict__ map, int total,
-                                             const float *__restrict__ x, float *xs)
+__device__ __forceinline__ void sb_stage_map(const SB_G int32_t *__restrict__ map, int total,
+                                             const SB_G float *__restrict__ x, float *xs)
 {
     for (int i0 = 0; i0 < total; i0 += 4 * WG) {
         int m[4];
@@ -274,16 +295,17 @@ __device__ __forceinline__ void sb_stage_map(const int32_t *__restrict__ map, in
 // The G lanes of a row are folded with a fixed xor butterfly and the
 // row-owner lane adds the sum to spy[row].
 template <int SU>
-__device__ __forceinline__ void sb_sparse_slice(const float *__restrict__ sval, const uint16_t *__restrict__ sidx,
-                                                const int32_t *__restrict__ tab, const int32_t *__restrict__ map,
+__device__ __forceinline__ void sb_sparse_slice(const SB_G float *__restrict__ sval,
+                                                const SB_G uint16_t *__restrict__ sidx,
+                                                const SB_G int32_t *__restrict__ tab, const SB_G int32_t *__restrict__ map,
                                                 int hs, int glog, int slice, int pad, const float *xs, float *spy)
 {
     const int lane = threadIdx.x & 63;
     const int off = tab[2 * slice];
     const int nst = tab[2 * slice + 1];
     if (nst <= 0) return;
-    const sb_f4 *__restrict__ V = reinterpret_cast<const sb_f4 *>(sval) + (int64_t)off * 64 + lane;
-    const sb_u2 *__restrict__ I = reinterpret_cast<const sb_u2 *>(sidx) + (int64_t)off * 64 + lane;
+    const SB_G sb_f4 *__restrict__ V = reinterpret_cast<const SB_G sb_f4 *>(sval) + (int64_t)off * 64 + lane;
+    const SB_G sb_u2 *__restrict__ I = reinterpret_cast<const SB_G sb_u2 *>(sidx) + (int64_t)off * 64 + lane;
     const unsigned padpair = (unsigned)pad | ((unsigned)pad << 16);
     sb_f4 cv[SU];
     sb_u2 ci[SU];
@@ -333,8 +355,8 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
     const int tpar = tile.dense_tpar;
     const int M = tile.dense_mcols;
     const SbMatBufs mb = sb_mat_bufs(a, tile.mat, a.xmap);
-    const float *__restrict__ x = mb.x;
-    float *__restrict__ y = mb.y;
+    const SB_G float *__restrict__ x = sb_gp(mb.x);
+    SB_G float *__restrict__ y = sb_gp(mb.y);
 
     SbStreamLane L;
     {
@@ -365,7 +387,7 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         head = false;
         if (slen > XCAP + 4) break;            // never built (the host and kernel choice check)
         __syncthreads();                       // previous group done with xs
-        sb_stage_map<WG>(mb.xmap + smap, slen, x, xs);
+        sb_stage_map<WG>(sb_gp(mb.xmap) + smap, slen, x, xs);
         __syncthreads();
         const int pe = first.grp_end;
         if (pe <= pb) break;                   // never built
@@ -373,13 +395,13 @@ __device__ __forceinline__ void sb_spmv_tile(const SbSpmvArgs &a, int t, float *
         for (int p = pb; p < pe; ++p) {
             const SbPiece pc = a.pieces[p];
             if (pc.kind == SB_PIECE_DENSE) {
-                sb_dense_piece<UNROLL>(reinterpret_cast<const sb_f4 *>(mb.dval + pc.data_off),
+                sb_dense_piece<UNROLL>(reinterpret_cast<const SB_G sb_f4 *>(sb_gp(mb.dval) + pc.data_off),
                                        (hs * pc.ncols + 3) >> 2, tpar, M, pc.ncols, xs + base, L, a0, a1, a2, a3);
             } else if (sw >= 0) {
                 // this wavefront's slices sw, sw + nsw, ...
                 for (int sl = sw; sl < nsl; sl += nsw) {
-                    sb_sparse_slice<SB_SPUNROLL>(mb.sval + pc.data_off, mb.sidx + pc.data_off,
-                                                 mb.srowptr + pc.rowptr_off, mb.srowptr + tile.sp_map, hs,
+                    sb_sparse_slice<SB_SPUNROLL>(sb_gp(mb.sval) + pc.data_off, sb_gp(mb.sidx) + pc.data_off,
+                                                 sb_gp(mb.srowptr) + pc.rowptr_off, sb_gp(mb.srowptr) + tile.sp_map, hs,
                                                  tile.sp_glog, sl, pc.ncols, xs + base,
                                                  spy);
                 }
@@ -439,9 +461,10 @@ __device__ __forceinline__ void sb_expand_xrow(const SbXRange *__restrict__ xr, 
 {
     int base = 0;
     for (int r = r0; r < r1; ++r) {
-        const SbXRange g = xr[r];
-        for (int i = threadIdx.x; i < g.len; i += blockDim.x) xrow[base + i] = g.col0 + i;
-        base += g.len;
+        const SB_G SbXRange *g = sb_gp(xr) + r;
+        const int col0 = g->col0, len = g->len;
+        for (int i = threadIdx.x; i < len; i += blockDim.x) xrow[base + i] = col0 + i;
+        base += len;
     }
 }
 
@@ -449,24 +472,24 @@ __device__ __forceinline__ void sb_expand_xrow(const SbXRange *__restrict__ xr, 
 // column c) into xs4[c * N/4 + v], four float4 loads in flight per thread,
 // coalesced along N.
 template <int WG>
-__device__ __forceinline__ void sb_stage_xrows(const float *__restrict__ X, int N, const int *xrow, int ncols,
+__device__ __forceinline__ void sb_stage_xrows(const SB_G float *__restrict__ X, int N, const int *xrow, int ncols,
                                                float4 *xs4)
 {
     const int NV = N >> 2;
     const int tot = ncols * NV;
     for (int i0 = 0; i0 < tot; i0 += 4 * WG) {
-        float4 v[4];
+        sb_f4 v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = i0 + u * WG + (int)threadIdx.x;
             const int c = i < tot ? i / NV : 0;
             const int q = i < tot ? i - c * NV : 0;
-            v[u] = *reinterpret_cast<const float4 *>(X + (size_t)xrow[c] * N + 4 * q);
+            v[u] = *reinterpret_cast<const SB_G sb_f4 *>(X + (size_t)xrow[c] * N + 4 * q);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = i0 + u * WG + (int)threadIdx.x;
-            if (i < tot) xs4[i] = v[u];
+            if (i < tot) reinterpret_cast<sb_f4 *>(xs4)[i] = v[u];
         }
     }
 }
@@ -518,7 +541,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPi
     const int NV = N >> 2;
     if (pc.kind == SB_PIECE_DENSE) {
         if (!m.active) return;
-        const float *__restrict__ P = a.dval + pc.data_off;
+        const SB_G float *__restrict__ P = sb_gp(a.dval) + pc.data_off;
         int c = 0;
         for (; c + 4 <= pc.ncols; c += 4) {
             float4 xv[4];
@@ -526,7 +549,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPi
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 xv[u] = xs4[(c + u) * NV + m.v];
-                const float *__restrict__ col = P + (size_t)(c + u) * hs;
+                const SB_G float *__restrict__ col = P + (size_t)(c + u) * hs;
 #pragma unroll
                 for (int i = 0; i < RTMAX; ++i) {
                     const int r = m.rbeg + i;
@@ -540,7 +563,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPi
         }
         for (; c < pc.ncols; ++c) {
             const float4 xv = xs4[c * NV + m.v];
-            const float *__restrict__ col = P + (size_t)c * hs;
+            const SB_G float *__restrict__ col = P + (size_t)c * hs;
 #pragma unroll
             for (int i = 0; i < RTMAX; ++i) {
                 const int r = m.rbeg + i;
@@ -550,9 +573,9 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPi
         return;
     }
     if (pc.kind != SB_PIECE_SPARSE) return;
-    const int32_t *__restrict__ rp = a.srowptr + pc.rowptr_off;
-    const float *__restrict__ sv = a.sval + pc.data_off;
-    const uint16_t *__restrict__ si = a.sidx + pc.data_off;
+    const SB_G int32_t *__restrict__ rp = sb_gp(a.srowptr) + pc.rowptr_off;
+    const SB_G float *__restrict__ sv = sb_gp(a.sval) + pc.data_off;
+    const SB_G uint16_t *__restrict__ si = sb_gp(a.sidx) + pc.data_off;
     const int nnzp = rp[hs];
     int rlo[RTMAX], rhi[RTMAX];
 #pragma unroll
@@ -598,8 +621,8 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
     const int hs = tile.nrows;
     const SbVecMap m = sb_vec_map(hs, N, WG);
     const SbMatBufs mb = sb_mat_bufs(a, tile.mat, nullptr);
-    const float *__restrict__ X = mb.x;
-    float *__restrict__ Y = mb.y;
+    const SB_G float *__restrict__ X = sb_gp(mb.x);
+    SB_G float *__restrict__ Y = sb_gp(mb.y);
     float4 acc[RTMAX];
 #pragma unroll
     for (int i = 0; i < RTMAX; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -618,11 +641,11 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
         for (int i = 0; i < RTMAX; ++i) {
             const int r = m.rbeg + i;
             if (i < m.rt && r < hs) {
-                float4 *dst = reinterpret_cast<float4 *>(Y + (size_t)(tile.row0 + r) * N + 4 * m.v);
-                float4 o = acc[i];
+                SB_G sb_f4 *dst = reinterpret_cast<SB_G sb_f4 *>(Y + (size_t)(tile.row0 + r) * N + 4 * m.v);
+                sb_f4 o = {acc[i].x, acc[i].y, acc[i].z, acc[i].w};
                 if (tile.y_accum) {      // the rows' MFMA tile wrote Y in the previous launch
-                    const float4 w = *dst;
-                    o = make_float4(w.x + o.x, w.y + o.y, w.z + o.z, w.w + o.w);
+                    const sb_f4 w = *dst;
+                    o = sb_f4{w.x + o.x, w.y + o.y, w.z + o.z, w.w + o.w};
                 }
                 *dst = o;
             }

@@ -45,12 +45,23 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 // k-steps of X loads in flight ahead of their staging (X rows are shared by
 // many tiles and mostly come from L2): divides SB_MFMA_DEPTH
 #ifndef SB_MFMA_XDEPTH
-#define SB_MFMA_XDEPTH 1
+#define SB_MFMA_XDEPTH 2
 #endif
 // 1: the issue order of a k-step is set explicitly (MFMAs interleaved with
 // the splits' VALU work); 0: the compiler's schedule
 #ifndef SB_MFMA_INTERLEAVE
 #define SB_MFMA_INTERLEAVE 0
+#endif
+// 16-row units to wavefronts: 0 round robin (unit u to wavefront u % NW),
+// 1 packed (UNITS consecutive units per wavefront: fewer wavefronts read
+// the staged X fragments, each for more units)
+#ifndef SB_MFMA_PACK
+#define SB_MFMA_PACK 0
+#endif
+// 1: one 16-row unit per wavefront and pass (fewer registers; a taller
+// tile takes more passes over its pieces)
+#ifndef SB_MFMA_UNITS1
+#define SB_MFMA_UNITS1 0
 #endif
 // waves per SIMD the MFMA kernels are compiled for: two (256 registers per
 // lane, accumulators included), so two 256-thread workgroups share a CU
@@ -137,7 +148,7 @@ __device__ __forceinline__ void sb_expand_xrow_mfma(const SbXRange *__restrict__
     for (int i = threadIdx.x; i < pad; i += blockDim.x) xrow[ncols + i] = 0;
 }
 
-template <int WG, int UPT, int SPLIT, int NRHS>
+template <int WG, int UPT, int SPLIT, int NRHS, bool MASK = true>
 __device__ __forceinline__ void sb_store_b(int N, int ncols, int ks, float (*f)[8], uint16_t *B, int term_stride)
 {
     constexpr bool EXACT = NRHS != 0 && UPT * WG == NRHS * 4;
@@ -151,7 +162,7 @@ __device__ __forceinline__ void sb_store_b(int N, int ncols, int ks, float (*f)[
         const int k0 = ks * 32 + kg * 8;
         float v[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = k0 + j < ncols ? f[uu][j] : 0.f;
+        for (int j = 0; j < 8; ++j) v[j] = !MASK || k0 + j < ncols ? f[uu][j] : 0.f;
         sb_bf16x8 t[SPLIT];
         sb_split8<SPLIT>(v, t);
 #pragma unroll
@@ -182,7 +193,7 @@ struct SbMfmaCfg {
     static constexpr int NMAX = NRHS ? NRHS : 256;
     static constexpr int NTMAX = NMAX / 16;
     static constexpr int NW = WG / 64;
-    static constexpr int UNITS = SB_MFMA_UNITS(NMAX);            // row tiles per wavefront and pass
+    static constexpr int UNITS = SB_MFMA_UNITS1 ? 1 : SB_MFMA_UNITS(NMAX);   // row tiles per wavefront and pass
     static constexpr int TERM_STRIDE = NMAX * SB_KP;             // bf16 elements per staged term
     static constexpr int UPT = (NMAX * 4 + WG - 1) / WG;         // X staging units per thread
     static constexpr int BUF_STRIDE = SPLIT * TERM_STRIDE;        // one staged k-step (bf16 elements)
@@ -190,10 +201,12 @@ struct SbMfmaCfg {
 };
 
 // Fixup of the MFMA row tiles split along k (a launch after the MFMA
-// launch): workgroup t takes fix tile t, whose ksplit chunks stored their
-// partial sums as (16-row unit, column tile) 1 KiB blocks in accumulator
-// order; it adds the chunks in chunk order 0, 1, ..., ksplit-1 (the same
-// order whatever order the chunks ran in) and writes the tile's rows.
+// launch): the workgroups (t, g), g < gridDim.y, take fix tile t, whose
+// ksplit chunks stored their partial sums as (16-row unit, column tile)
+// 1 KiB blocks in accumulator order; a wavefront takes every
+// (gridDim.y * WG / 64)-th block, loads its chunks four at a time and adds
+// them in chunk order 0, 1, ..., ksplit-1 (the same order whatever order
+// the chunks ran in), then writes the block's rows.
 template <int WG>
 __device__ __forceinline__ void sb_mfma_fixup_body(const SbTile *fix_tiles, int nfix, const SbMatBufs *mats,
                                                    const float *part0, float *y0, int N)
@@ -201,16 +214,26 @@ __device__ __forceinline__ void sb_mfma_fixup_body(const SbTile *fix_tiles, int 
     const int t = blockIdx.x;
     if (t >= nfix) return;
     const SbTile tile = fix_tiles[t];
-    const float *part = mats ? mats[tile.mat].part : part0;
-    float *Y = mats ? mats[tile.mat].y : y0;
+    const SB_G float *part = sb_gp(mats ? mats[tile.mat].part : part0);
+    SB_G float *Y = sb_gp(mats ? mats[tile.mat].y : y0);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int nt = N >> 4;
     const int nblk = ((tile.nrows + 15) >> 4) * nt;
-    const sb_f32x4 *__restrict__ P = reinterpret_cast<const sb_f32x4 *>(part) + (int64_t)tile.part_blk * 64;
-    for (int b = wave; b < nblk; b += WG / 64) {
+    const int nwav = gridDim.y * (WG / 64);
+    const int ks = tile.ksplit;
+    const SB_G sb_f32x4 *__restrict__ P = reinterpret_cast<const SB_G sb_f32x4 *>(part) + (int64_t)tile.part_blk * 64;
+    for (int b = blockIdx.y * (WG / 64) + wave; b < nblk; b += nwav) {
         sb_f32x4 s = P[(int64_t)b * 64 + lane];
-        for (int c = 1; c < tile.ksplit; ++c) s += P[((int64_t)c * nblk + b) * 64 + lane];
+        for (int c = 1; c < ks; c += 4) {
+            sb_f32x4 v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (c + i < ks) v[i] = P[((int64_t)(c + i) * nblk + b) * 64 + lane];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (c + i < ks) s += v[i];
+        }
         const int m = b / nt, j = b - m * nt;
         const int rbase = m * 16 + 4 * (lane >> 4);
         const int col = j * 16 + (lane & 15);
@@ -218,6 +241,20 @@ __device__ __forceinline__ void sb_mfma_fixup_body(const SbTile *fix_tiles, int 
         for (int r = 0; r < 4; ++r)
             if (rbase + r < tile.nrows) Y[(size_t)(tile.row0 + rbase + r) * N + col] = s[r];
     }
+}
+
+// Workgroups per fix tile of the fixup launch: enough 4-wavefront
+// workgroups for one block per wavefront of the tallest fix tile (at most
+// 64).
+inline int sb_fixup_groups(const SbTile *fix, size_t nfix, int n_rhs)
+{
+    int most = 0;
+    for (size_t i = 0; i < nfix; ++i) {
+        const int nblk = ((fix[i].nrows + 15) >> 4) * (n_rhs >> 4);
+        most = nblk > most ? nblk : most;
+    }
+    const int g = (most + 3) / 4;
+    return g < 1 ? 1 : g > 64 ? 64 : g;
 }
 
 // The k-loop of one MFMA piece for a wavefront holding NU units (16-row
@@ -248,7 +285,10 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
     const int nb = lane & 15, kb = (lane >> 4) * 8;
     const float *__restrict__ X = mb.x;
     const int K = pc.ksteps;
-    const sb_f4 *__restrict__ A = reinterpret_cast<const sb_f4 *>(mb.dval + pc.data_off);
+    // a global-address-space pointer: loads through a generic pointer would
+    // be FLAT loads, which count against the LDS counter as well, so every
+    // wait for a B fragment would also wait for the A loads in flight
+    const SB_G sb_f4 *__restrict__ A = reinterpret_cast<const SB_G sb_f4 *>(sb_gp(mb.dval) + pc.data_off);
     float xf[DX][Cfg::UPT][8];
     sb_f4 ar[D][NUA][2];
     sb_bf16x8 at[NUA][SPLIT];
@@ -291,8 +331,11 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
     __syncthreads();
     // k-step ks: `at` holds its A terms; A slot sl (= ks % D) holds the raw
     // A of k-step ks + D, slot (sl + 1) % D that of k-step ks + 1
-    auto step = [&](auto slc, const int ks) {
+    // MASK: the step stages the last k-step, whose columns past the piece
+    // are zeroed (only the last two steps of the piece)
+    auto step = [&](auto slc, auto maskc, const int ks) {
         constexpr int sl = decltype(slc)::value;
+        constexpr bool MASK = decltype(maskc)::value != 0;
         const uint16_t *Bc = B + (ks & 1) * Cfg::BUF_STRIDE;
         if constexpr (NU > 0) {
             using Pt = SbPasses<SPLIT>;
@@ -327,8 +370,8 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
         // this writes zeros nobody reads), then load the X values of
         // k-step ks + 1 + DX into the slot it frees
         constexpr int xs = (sl + 1) % DX;
-        sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, ks + 1, xf[xs], B + ((ks + 1) & 1) * Cfg::BUF_STRIDE,
-                                              Cfg::TERM_STRIDE);
+        sb_store_b<WG, Cfg::UPT, SPLIT, NRHS, MASK>(N, pc.ncols, ks + 1, xf[xs],
+                                                    B + ((ks + 1) & 1) * Cfg::BUF_STRIDE, Cfg::TERM_STRIDE);
         sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1 + DX < K ? ks + 1 + DX : K - 1, xf[xs]);
 #if SB_MFMA_INTERLEAVE
         // issue order of the step: the first column pair's B reads, then
@@ -347,16 +390,20 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
 #endif
         __syncthreads();
     };
+    // main loop: the steps that stage k-steps below K - 1 (every column in
+    // the piece); then the last (at most D + 1) steps with the mask
     int ks = 0;
-    for (; ks + D <= K; ks += D) {
-        step(SbInt<0>{}, ks);
-        if constexpr (D > 1) step(SbInt<(D > 1 ? 1 : 0)>{}, ks + 1);
-        if constexpr (D > 2) step(SbInt<(D > 2 ? 2 : 0)>{}, ks + 2);
-        if constexpr (D > 3) step(SbInt<(D > 3 ? 3 : 0)>{}, ks + 3);
+    for (; ks + D + 1 < K; ks += D) {
+        step(SbInt<0>{}, SbInt<0>{}, ks);
+        if constexpr (D > 1) step(SbInt<(D > 1 ? 1 : 0)>{}, SbInt<0>{}, ks + 1);
+        if constexpr (D > 2) step(SbInt<(D > 2 ? 2 : 0)>{}, SbInt<0>{}, ks + 2);
+        if constexpr (D > 3) step(SbInt<(D > 3 ? 3 : 0)>{}, SbInt<0>{}, ks + 3);
     }
-    if constexpr (D > 1) if (ks < K) step(SbInt<0>{}, ks);
-    if constexpr (D > 2) if (ks + 1 < K) step(SbInt<(D > 2 ? 1 : 0)>{}, ks + 1);
-    if constexpr (D > 3) if (ks + 2 < K) step(SbInt<(D > 3 ? 2 : 0)>{}, ks + 2);
+    if (ks < K) step(SbInt<0>{}, SbInt<1>{}, ks);
+    if constexpr (D > 1) if (ks + 1 < K) step(SbInt<(D > 1 ? 1 : 0)>{}, SbInt<1>{}, ks + 1);
+    if constexpr (D > 2) if (ks + 2 < K) step(SbInt<(D > 2 ? 2 : 0)>{}, SbInt<1>{}, ks + 2);
+    if constexpr (D > 3) if (ks + 3 < K) step(SbInt<(D > 3 ? 3 : 0)>{}, SbInt<1>{}, ks + 3);
+    if (ks + D < K) step(SbInt<0>{}, SbInt<1>{}, ks + D);
 }
 
 // One pass of a wavefront holding NU units umi[0..NU) over every piece of
@@ -393,12 +440,13 @@ __device__ __forceinline__ void sb_mfma_pass(const SbSpmmArgs &a, const SbMatBuf
             if (NRHS || jj < nt) {
                 if (split) {
                     const int64_t blk = (int64_t)tile.part_blk + (int64_t)(tile.kpart * mt + umi[i]) * nt + jj;
-                    reinterpret_cast<sb_f32x4 *>(mb.part)[blk * 64 + lane] = acc[i][jj];
+                    reinterpret_cast<SB_G sb_f32x4 *>(sb_gp(mb.part))[blk * 64 + lane] = acc[i][jj];
                 } else {
                     const int col = jj * 16 + (lane & 15);
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (rbase + r < tile.nrows) mb.y[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
+                        if (rbase + r < tile.nrows)
+                            sb_gp(mb.y)[(size_t)(tile.row0 + rbase + r) * N + col] = acc[i][jj][r];
                 }
             }
         }
@@ -425,22 +473,24 @@ __device__ __forceinline__ void sb_spmm_mfma_tile(const SbSpmmArgs &a, int t, in
     const SbTile tile = a.tiles[t];
     const int mt = (tile.nrows + 15) >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int mine = mt > wave ? (mt - wave + NW - 1) / NW : 0;   // units of this wavefront
+    const int mine = mt > wave ? (mt - wave + NW - 1) / NW : 0;   // units of this wavefront (round robin)
     const SbMatBufs mb = sb_mat_bufs(a, tile.mat, nullptr);
     uint16_t *B = reinterpret_cast<uint16_t *>(smem);
     // every wavefront runs the same number of passes (the staging barriers)
     int all_passes = (((mt + NW - 1) / NW) + UN - 1) / UN;
     if (all_passes < 1) all_passes = 1;
     for (int pass = 0; pass < all_passes; ++pass) {
-        // this pass's units: index i -> unit wave + NW * (pass * UN + i);
-        // the active ones are a prefix of nact
+        // this pass's units: index i -> unit wave + NW * (pass * UN + i)
+        // (round robin) or NW * UN * pass + UN * wave + i (packed); the
+        // active ones are a prefix of nact
         int umi[UN];
         int nact = 0;
 #pragma unroll
         for (int i = 0; i < UN; ++i) {
             const int k = pass * UN + i;
-            const bool act = k < mine;
-            umi[i] = act ? wave + NW * k : 0;
+            const int u = SB_MFMA_PACK ? (pass * NW + wave) * UN + i : wave + NW * k;
+            const bool act = SB_MFMA_PACK ? u < mt : k < mine;
+            umi[i] = act ? u : 0;
             nact += act;
         }
         if (nact >= UN)

@@ -1,8 +1,8 @@
 """Route selection: the learned MLP router and the keep-if-faster rule.
 
-``Router.route_blocks`` predicts the per-block cost of every applicable route
-(dense FP32, sparse, and MFMA when the RHS width allows it) and picks the
-cheapest.  ``choose_plan`` builds the fixed-threshold plan (the fallback,
+``Router.route_and_shape`` predicts the per-block cost of every applicable route
+(dense FP32, sparse, and MFMA when the RHS width allows it and both block
+edges reach 16) under every tile shape, and picks the cheapest.  ``choose_plan`` builds the fixed-threshold plan (the fallback,
 threshold 15 as in the reference's mixed flows) and the learned plans, checks
 them against the same FP64 tolerance, times them cold, and keeps the fastest
 learned plan only if it beats the fixed one.
@@ -24,6 +24,7 @@ import os
 import numpy as np
 
 from .. import convert
+from ..vbr import block_table
 from .features import block_features, encode
 
 ROUTES = (convert.ROUTE_DENSE, convert.ROUTE_SPARSE, convert.ROUTE_DENSE_MFMA)
@@ -34,10 +35,23 @@ BIAS_GRID = {"spmv": ((0.0, 0.0), (0.5, 0.0), (1.0, 0.0)),
              "spmm": ((0.0, 0.0), (0.0, 0.5), (0.0, 1.0))}
 
 
-def allowed_mask(n_rhs, nblocks):
+# MFMA tiles are 16 rows by 32 columns: a block goes to MFMA only when both
+# of its edges reach 16 (shorter edges would be mostly padding)
+MFMA_MIN_EDGE = 16
+
+
+def mfma_edges_ok(rows, cols):
+    return (np.asarray(rows) >= MFMA_MIN_EDGE) & (np.asarray(cols) >= MFMA_MIN_EDGE)
+
+
+def allowed_mask(n_rhs, nblocks, rows=None, cols=None):
+    """[nblocks, 3] routes each block may take: MFMA needs N a multiple of 16
+    up to 256 and, when the block edges are given, both edges >= 16."""
     m = np.ones((nblocks, 3), dtype=bool)
     if n_rhs == 1 or n_rhs % 16 or n_rhs > 256:
         m[:, 2] = False
+    elif rows is not None:
+        m[:, 2] = mfma_edges_ok(rows, cols)
     return m
 
 
@@ -73,7 +87,7 @@ class Router:
         pred = self.predict_costs(F).astype(np.float64).reshape(len(F), self.n_shapes, 3)
         pred[:, :, 1] -= float(sparse_bias)
         pred[:, :, 2] -= float(mfma_bias)
-        pred[np.broadcast_to(~allowed_mask(n_rhs, len(F))[:, None, :], pred.shape)] = np.inf
+        pred[np.broadcast_to(~allowed_mask(n_rhs, len(F), F[:, 0], F[:, 1])[:, None, :], pred.shape)] = np.inf
         return pred
 
     def route_and_shape(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,
@@ -111,7 +125,8 @@ class Router:
 def fixed_routes(vbr, n_rhs=1, threshold=convert.DEFAULT_DENSITY, mfma=False):
     r = convert.threshold_routes(vbr.val, vbr.indx, threshold)
     if mfma and n_rhs > 1 and n_rhs % 16 == 0 and n_rhs <= 256:
-        r = np.where(r == convert.ROUTE_DENSE, convert.ROUTE_DENSE_MFMA, r).astype(np.uint8)
+        _, _, _, h, _, w = block_table(vbr.rpntr, vbr.cpntr, vbr.bindx, vbr.bpntrb, vbr.bpntre)
+        r = np.where((r == convert.ROUTE_DENSE) & mfma_edges_ok(h, w), convert.ROUTE_DENSE_MFMA, r).astype(np.uint8)
     return r
 
 

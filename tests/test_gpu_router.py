@@ -78,38 +78,44 @@ def _c3_like(n, seed):
 
 def test_trained_router_gains_over_fixed_threshold(mods):
     """What the trained network gains: on a 30k x 30k SpMM (N = 128) shaped
-    like the 50k workload, the committed router's plan is within 1e-5 of the
-    FP64 oracle like the fixed-threshold plans (T = 15) and at least 10%
-    faster cold (median of 15 launches) than the reference's policy (T = 15,
-    dense blocks on the FP32 path).  The fixed policy with the dense blocks
-    on MFMA is timed beside it (printed, not asserted: it can be as fast;
-    choose_plan keeps whichever is faster)."""
+    like the 50k workload, the committed router's plan (its routes and its
+    tile shape) is within 1e-5 of the FP64 oracle like the fixed-threshold
+    plans (T = 15), at least 10% faster cold (median of 15 launches) than
+    the reference's policy (T = 15, dense blocks on the FP32 path), and
+    faster than the same fixed policy with its dense blocks on MFMA in
+    either tile shape -- the gain beyond the route set itself."""
     rt, _ = mods
     from oracle import c_oracle
     from sable_amd.convert import threshold_routes, ROUTE_DENSE, ROUTE_DENSE_MFMA
     from sable_amd.router import Router
+    from sable_amd.router.labels import SHAPES, shape_wg
     assert Router.available()
     vbr, X = _c3_like(30000, 21)
     fixed = threshold_routes(vbr.val, vbr.indx, 15).astype(np.uint8)
     fixed_mfma = np.where(fixed == ROUTE_DENSE, ROUTE_DENSE_MFMA, fixed).astype(np.uint8)
-    learned = Router.load().route_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                         vbr.bpntre, 128)
+    learned, shape, wg = Router.load().route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                       vbr.bpntrb, vbr.bpntre, 128)
     Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
                                              vbr.bpntrb, vbr.bpntre, X)
     Xd = rt.DeviceArray.from_numpy(X)
     Yd = rt.DeviceArray((X.shape[0], 128))
+    cases = [("fixed", fixed, None), ("learned", learned, wg)]
+    cases += [(f"fixed_mfma_wg{shape_wg('spmm', s)}", fixed_mfma, shape_wg("spmm", s)) for s in range(len(SHAPES["spmm"]))]
     out = {}
-    for name, routes in (("fixed", fixed), ("fixed_mfma", fixed_mfma), ("learned", learned)):
+    for name, routes, w in cases:
         print(f"case: router gain, {name} plan", flush=True)
-        p = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=128)
+        kw = {} if w is None else {"wg": w}
+        p = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=128, **kw)
         t = float(np.median(p.time(Xd, Yd, warmup=3, reps=15, flush_bytes=1 << 30)))
         err = c_oracle.scaled_error(Yd.numpy().ravel(), Y64.ravel(), AY.ravel())["componentwise"]
         out[name] = (t, err)
         p.close()
-    print(f"fixed {out['fixed'][0]:.4f} ms, fixed_mfma {out['fixed_mfma'][0]:.4f} ms, "
-          f"learned {out['learned'][0]:.4f} ms", flush=True)
+    print("router gain (ms, err): " + ", ".join(f"{k} {v[0]:.4f} {v[1]:.1e}" for k, v in out.items())
+          + f"; learned shape {shape} (wg {wg})", flush=True)
     assert all(e <= 1e-5 for _, e in out.values())
     assert out["learned"][0] < 0.9 * out["fixed"][0]
+    best_fixed_mfma = min(v[0] for k, v in out.items() if k.startswith("fixed_mfma"))
+    assert out["learned"][0] < best_fixed_mfma
 
 
 def test_choose_plan_keeps_the_faster_plan(mods):
