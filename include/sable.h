@@ -162,6 +162,18 @@ int sable_spmv(SablePlan *plan, const float *x, float *y, int flags);
  * as a parameter.  Y = A X, X [ncols x N] and Y [nrows x N] row-major. */
 int sable_spmm(SablePlan *plan, const float *X, float *Y, int flags);
 
+/* No counterpart in the reference: the variant of sable_spmm for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Y = A X with device X/Y (host pointers
+ * are refused), bit-identical to sable_spmm on the same plan; it calls no
+ * allocator: the partial sums of the plan's k-split MFMA tiles go to the
+ * library's file-scope device scratch array (64 MiB, held by every GPU the
+ * library runs on until the process ends) instead of the plan's buffer.  A
+ * plan needing more scratch (SablePlanStats.scratch_bytes) is refused with
+ * SABLE_ERR_UNSUPPORTED.  Calls that overlap on one GPU share the array and
+ * race. */
+int sable_spmm_noalloc(SablePlan *plan, const float *X, float *Y, int flags);
+
 /* Replaces: the timed loop of src/codegen.py:414-439 and eval.py:23-45.
  * Runs `warmup` untimed launches then `reps` launches, each timed by HIP
  * events attached to the kernel launch itself (its start and end, without

@@ -96,6 +96,7 @@ _SIGS = {
     "sable_plan_kernel_spec": (I32, [P, C.POINTER(I32)]),
     "sable_spmv": (I32, [P, P, P, I32]),
     "sable_spmm": (I32, [P, P, P, I32]),
+    "sable_spmm_noalloc": (I32, [P, P, P, I32]),
     "sable_time": (I32, [P, P, P, I32, I32, I64, F32P]),
     "sable_group_create": (I32, [C.POINTER(P), I32, C.POINTER(P), C.POINTER(P), I32, I32, C.POINTER(P)]),
     "sable_group_launch": (I32, [P, I32]),

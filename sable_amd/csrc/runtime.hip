@@ -106,12 +106,21 @@ extern "C" __global__ void __launch_bounds__(256) sb_copy_probe(const sb_f4 *__r
 // embedded kernel sources for hipRTC (generated at build time)
 #include "_embedded_src.inc"
 
+// Scratch of sable_spmm_noalloc: the partial sums of a plan's split MFMA
+// tiles, in a file-scope device array that every GPU the library runs on
+// holds until the process ends.  Host code never passes the
+// array's name to a kernel: it is a host-side stand-in; its device address
+// per GPU comes from hipGetSymbolAddress in sable_init.
+#define SABLE_NOALLOC_SCRATCH_BYTES (64LL << 20)
+static __device__ __align__(256) unsigned char scratch[SABLE_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
 std::mutex g_mu;
 int g_device = -1;
 int g_ncu = 0;
+void *g_scratch[64] = {};   // device address of `scratch` per GPU (set by sable_init)
 int64_t g_hbm = 0;
 std::string g_arch = "gfx950";
 std::string g_cache_dir;
@@ -636,6 +645,12 @@ int sable_init(int device)
     HIPOK(hipGetDeviceProperties(&prop, device));
     g_device = device;
     g_ncu = prop.multiProcessorCount;
+    if (device < 64 && !g_scratch[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return fail(SABLE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = ptr;
+    }
     g_hbm = (int64_t)prop.totalGlobalMem;
     std::string arch = prop.gcnArchName;
     const size_t colon = arch.find(':');
@@ -865,6 +880,28 @@ int sable_spmm(SablePlan *p, const float *X, float *Y, int flags)
     if (p && p->host.op != SABLE_OP_SPMM) return fail(SABLE_ERR_INVALID, "plan was built for SpMV");
     return run_op(p, X, Y, flags, p ? (int64_t)p->stats.ncols * p->stats.n_rhs : 0,
                   p ? (int64_t)p->stats.nrows * p->stats.n_rhs : 0);
+}
+
+// Y = A X like sable_spmm with device X/Y, allocating nothing: the split MFMA
+// tiles' partial sums live in the file-scope `scratch` array (through its
+// per-device address) instead of the plan's own buffer.  Nothing in it needs
+// to start at zero: every partial is written before the fixup reads it in
+// the same call.  Calls that overlap on one GPU share the array and race.
+int sable_spmm_noalloc(SablePlan *p, const float *X, float *Y, int flags)
+{
+    if (!p || p->host_only) return fail(SABLE_ERR_INVALID, "invalid plan");
+    if (p->host.op != SABLE_OP_SPMM) return fail(SABLE_ERR_INVALID, "plan was built for SpMV");
+    if (!X || !Y) return fail(SABLE_ERR_INVALID, "null X/Y");
+    if (flags & SABLE_HOST_PTRS) return fail(SABLE_ERR_INVALID, "sable_spmm_noalloc takes device pointers only");
+    if (g_device < 0 || g_device >= 64 || !g_scratch[g_device]) return fail(SABLE_ERR_NO_DEVICE, "sable_init not called");
+    const int64_t need = p->host.part_blocks * 1024;
+    if (need > SABLE_NOALLOC_SCRATCH_BYTES)
+        return fail(SABLE_ERR_UNSUPPORTED, "plan needs " + std::to_string(need) + " bytes of scratch; "
+                    "sable_spmm_noalloc holds at most " + std::to_string(SABLE_NOALLOC_SCRATCH_BYTES));
+    CHECKRC(plan_launch_with(p, X, Y, (float *)g_scratch[g_device]));
+    HIPOK(hipGetLastError());
+    if (!(flags & SABLE_NO_SYNC)) HIPOK(hipStreamSynchronize(p->stream));
+    return SABLE_OK;
 }
 
 int sable_time(SablePlan *p, const float *x, float *y, int warmup, int reps, int64_t flush_bytes,
