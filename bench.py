@@ -147,7 +147,9 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
     the FP32 vector path; MFMA = the same routes with dense blocks on the
     bf16-split MFMA path (3-term split, 6 passes; and hi+lo, 3 passes);
     learned = the router's plans at MFMA-route biases 0, 0.5 and 1 (3-term
-    split), each in the workgroup size the router picks.  The fastest learned
+    split), each under every tile shape of the router and in the default
+    workgroup size (its own shape pick marked); every-block-on-MFMA is timed
+    beside them as a fixed policy.  The fastest learned
     plan within 1e-5 of FP64 is kept if faster than MFMA x6 on the threshold
     routes; the kept plan is then timed again with fresh launches.  Returns
     the result dict (roofline of the kept plan, every variant's time and
@@ -168,19 +170,33 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
     from sable_amd.router import Router
     from sable_amd.router.router import BIAS_GRID
     if args.routing in ("learned", "both") and Router.available():
+        from sable_amd.autotune import tuned_options
         router = Router.load()
+        default_wg = tuned_options("spmm", n_rhs).get("wg") or 256
+        seen = []
         for sb, mb in BIAS_GRID["spmm"]:
-            # the router's routes and tile shape (workgroup size)
-            lr, _, wg = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                               vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
-            variants.append((f"learned_x6 mfma_bias={mb:g} wg={wg}", lr, 3, {"wg": wg, "mfma_bias": mb}))
+            # the router's routes under each tile shape (workgroup size), its
+            # own pick marked, and the pick's routes in the default workgroup
+            # size when no shape has it
+            shapes, pick = router.routes_per_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                   vbr.bpntrb, vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
+            cands = [(r, w, s == pick) for s, (r, w) in enumerate(shapes)]
+            if all(w != default_wg for _, w in shapes):
+                cands.append((shapes[pick][0], default_wg, False))
+            for lr, wg, is_pick in cands:
+                if any(np.array_equal(lr, r) and wg == w for r, w in seen):
+                    continue
+                seen.append((lr, wg))
+                variants.append((f"learned_x6 mfma_bias={mb:g} wg={wg}", lr, 3,
+                                 {"wg": wg, "mfma_bias": mb, "router_pick": is_pick}))
     if args.spmm_variants:
         keep = [v for v in args.spmm_variants.split(",") if v]
         variants = [v for v in variants if any(v[0] == k or v[0].startswith(k + " ") for k in keep)]
     plans = {}
     for name, routes, split, kw in variants:
         kw = dict(kw)
-        meta = {"mfma_bias": kw.pop("mfma_bias", None), "split": split}
+        meta = {"mfma_bias": kw.pop("mfma_bias", None), "router_pick": kw.pop("router_pick", None),
+                "split": split}
         t1 = time.time()
         plan = runtime.Plan.from_vbr(vbr, routes.astype(np.uint8), op="spmm", n_rhs=n_rhs, mfma_split=split, **kw)
         insp = time.time() - t1
@@ -241,7 +257,8 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
                          "gflops": v["stats"]["flops_alg"] / (v["ms"] * 1e-3) / 1e9,
                          "inspector_s": v["inspector_s"], "bytes_alg": v["stats"]["bytes_alg"],
                          "ntiles": v["stats"]["ntiles"], "ntiles_mfma": v["stats"]["ntiles_mfma"],
-                         "wg": v["stats"]["wg"], "split": v["split"], "mfma_bias": v["mfma_bias"]}
+                         "wg": v["stats"]["wg"], "split": v["split"], "mfma_bias": v["mfma_bias"],
+                         "router_pick": v["router_pick"]}
                      for k, v in res.items()},
         "bytes_alg": st["bytes_alg"], "flops_alg": st["flops_alg"], "cpu_baseline": None,
     }
@@ -777,17 +794,24 @@ def main():
             # one learned plan per sparse-route bias; the fastest within 1e-5 is "learned"
             cands = []
             for sb, mb in BIAS_GRID["spmv"]:
-                lr, _, wgl = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
-                                                    vbr.bpntrb, vbr.bpntre, 1, sparse_bias=sb, mfma_bias=mb)
-                r = measure(lr, f"learned sparse_bias={sb:g}", wg=wgl)
-                r["sparse_bias"] = sb
-                r["routes_changed"] = int(np.sum(lr != routes))
-                r["label"] = f"learned sparse_bias={sb:g}"
-                cands.append(r)
+                # the router's routes under each tile shape (workgroup
+                # size); its own shape pick is marked, not trusted: the
+                # per-block costs do not see the plan's tile count
+                shapes, pick = router.routes_per_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                       vbr.bpntrb, vbr.bpntre, 1, sparse_bias=sb, mfma_bias=mb)
+                for s_i, (lr, wgl) in enumerate(shapes):
+                    lab = f"learned sparse_bias={sb:g} wg={wgl}"
+                    r = measure(lr, lab, wg=wgl)
+                    r["sparse_bias"] = sb
+                    r["wg"] = wgl
+                    r["router_pick"] = s_i == pick
+                    r["routes_changed"] = int(np.sum(lr != routes))
+                    r["label"] = lab
+                    cands.append(r)
             good = [r for r in cands if r["error_vs_fp64"]["componentwise"] <= 1e-5]
             res["learned"] = min(good or cands, key=lambda r: r["ms"])
-            res["learned_candidates"] = [{k: r[k] for k in ("sparse_bias", "ms", "median_ms", "bytes_alg")}
-                                         for r in cands]
+            res["learned_candidates"] = [{k: r[k] for k in ("sparse_bias", "wg", "router_pick", "ms", "median_ms",
+                                                            "bytes_alg")} for r in cands]
             ok = res["learned"]["error_vs_fp64"]["componentwise"] <= 1e-5
             # keep-if-faster at identical tolerance
             if args.routing == "learned" or (ok and res["learned"]["ms"] < res["plain"]["ms"]):
@@ -839,7 +863,9 @@ def main():
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc,
                    "routing": (f"{kept} (learned = fastest of the router's plans at sparse biases "
-                               f"{[c['sparse_bias'] for c in res['learned_candidates']]}, kept only if faster "
+                               f"{sorted(set(c['sparse_bias'] for c in res['learned_candidates']))} under its "
+                               f"tile shapes (workgroup sizes "
+                               f"{sorted(set(c['wg'] for c in res['learned_candidates']))}), kept only if faster "
                                f"at 1e-5, then timed again with fresh launches; plain = fixed threshold "
                                f"T={args.threshold:g})" if "learned" in res
                                else f"plain fixed threshold T={args.threshold:g}"),

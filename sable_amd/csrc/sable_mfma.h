@@ -63,6 +63,10 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 #ifndef SB_MFMA_UNITS1
 #define SB_MFMA_UNITS1 0
 #endif
+// 1: a scheduling fence after each column pair's MFMAs
+#ifndef SB_MFMA_FENCE
+#define SB_MFMA_FENCE 0
+#endif
 // waves per SIMD the MFMA kernels are compiled for: two (256 registers per
 // lane, accumulators included), so two 256-thread workgroups share a CU
 #ifndef SB_MFMA_WAVES
@@ -357,6 +361,11 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
                                     acc[i][jp * JB + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                                         at[i][Pt::a(p)], bt[jp & 1][c][Pt::b(p)], acc[i][jp * JB + c], 0, 0, 0);
                 }
+#if SB_MFMA_FENCE
+                // keeps the next pairs' B reads from being hoisted above
+                // these MFMAs (register budget of the higher occupancies)
+                __builtin_amdgcn_sched_barrier(0);
+#endif
             }
             // the next k-step's A terms, then the slot refilled D ahead
             constexpr int nsl = (sl + 1) % D;

@@ -114,6 +114,22 @@ class Router:
         routes = np.asarray(ROUTES, dtype=np.uint8)[best[:, s]]
         return routes, s, shape_wg(op, s)
 
+    def routes_per_shape(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,
+                         mfma_bias=0.0):
+        """The routes under every tile shape and the shape the router picks
+        (the one route_and_shape returns): ([(routes, workgroup size) per
+        shape], picked shape index)."""
+        from .labels import shape_wg
+        F = block_features(val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs)
+        op = "spmv" if n_rhs == 1 else "spmm"
+        if len(F) == 0:
+            return [(np.zeros(0, dtype=np.uint8), shape_wg(op, s)) for s in range(self.n_shapes)], 0
+        pred = self._costs(F, n_rhs, sparse_bias, mfma_bias)
+        best = np.argmin(pred, axis=2)
+        tot = np.sum(np.exp2(np.take_along_axis(pred, best[:, :, None], axis=2)[:, :, 0]), axis=0)
+        table = np.asarray(ROUTES, dtype=np.uint8)
+        return [(table[best[:, s]], shape_wg(op, s)) for s in range(self.n_shapes)], int(np.argmin(tot))
+
     def route_blocks(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,
                      mfma_bias=0.0, shape=None):
         """The cheapest allowed route per block (under the chosen shape, see
@@ -133,13 +149,19 @@ def fixed_routes(vbr, n_rhs=1, threshold=convert.DEFAULT_DENSITY, mfma=False):
 def choose_plan(vbr, x, router=None, n_rhs=1, threshold=convert.DEFAULT_DENSITY, tol=1e-5, reps=20,
                 flush_bytes=1 << 30, biases=None, **plan_kw):
     """Builds the fixed-threshold plan (for SpMM with the dense blocks on
-    MFMA when N allows it) and, when a router is available, one learned
-    plan per (sparse_bias, mfma_bias) in ``biases`` (default
-    ``BIAS_GRID[op]``; candidates giving the same routes are built once);
-    returns (plan, report).  The fastest learned plan is kept only if its
-    error against the FP64 oracle is within ``tol`` like the fixed plan and
-    its cold median launch time is lower."""
+    MFMA when N allows it) and, when a router is available, the learned
+    plans: per (sparse_bias, mfma_bias) in ``biases`` (default
+    ``BIAS_GRID[op]``), the router's routes under every tile shape, plus
+    the routes of the shape the router picks in the plan's default
+    workgroup size when no shape has it (candidates giving the same routes
+    and workgroup size are built once); returns (plan, report).  The
+    router's per-block costs do not see a plan's tile count, so its shape
+    pick is timed beside the others, not trusted (``router_pick`` marks it
+    in the report).  The fastest learned plan is kept only if its error
+    against the FP64 oracle is within ``tol`` like the fixed plan and its
+    cold median launch time is lower."""
     from ..runtime import Plan, DeviceArray
+    from ..autotune import tuned_options
     op = "spmv" if n_rhs == 1 else "spmm"
     xd = DeviceArray.from_numpy(np.asarray(x, dtype=np.float32))
     shape_y = (int(vbr.rpntr[-1]),) if n_rhs == 1 else (int(vbr.rpntr[-1]), n_rhs)
@@ -161,30 +183,37 @@ def choose_plan(vbr, x, router=None, n_rhs=1, threshold=convert.DEFAULT_DENSITY,
     if router is None:
         return p_fixed, report
     fixed_ok = _within(vbr, x, y_fixed, n_rhs, tol)
+    default_wg = plan_kw.get("wg") or tuned_options(op, n_rhs).get("wg") or 256
     best = None
     seen = []
     report["candidates"] = []
     for sb, mb in (BIAS_GRID[op] if biases is None else biases):
-        learned, _, wg = router.route_and_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
-                                                vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
-        if any(np.array_equal(learned, r) and wg == w for r, w in seen):
-            continue
-        seen.append((learned, wg))
-        p, t, y = run(learned, wg)
-        ok = fixed_ok and _within(vbr, x, y, n_rhs, tol)
-        report["candidates"].append({"sparse_bias": sb, "mfma_bias": mb, "wg": wg, "ms": t, "within_tol": bool(ok),
-                                     "routes": {int(k): int(v) for k, v in
-                                                zip(*np.unique(learned, return_counts=True))}})
-        if ok and (best is None or t < best[1]):
-            if best is not None:
-                best[0].close()
-            best = (p, t, sb, mb, learned, wg)
-        else:
-            p.close()
+        shapes, pick = router.routes_per_shape(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb,
+                                               vbr.bpntre, n_rhs, sparse_bias=sb, mfma_bias=mb)
+        cands = [(r, w, s == pick) for s, (r, w) in enumerate(shapes)]
+        if all(w != default_wg for _, w in shapes):
+            cands.append((shapes[pick][0], default_wg, False))
+        for learned, wg, is_pick in cands:
+            if any(np.array_equal(learned, r) and wg == w for r, w in seen):
+                continue
+            seen.append((learned, wg))
+            p, t, y = run(learned, wg)
+            ok = fixed_ok and _within(vbr, x, y, n_rhs, tol)
+            report["candidates"].append({"sparse_bias": sb, "mfma_bias": mb, "wg": wg, "router_pick": bool(is_pick),
+                                         "ms": t, "within_tol": bool(ok),
+                                         "routes": {int(k): int(v) for k, v in
+                                                    zip(*np.unique(learned, return_counts=True))}})
+            if ok and (best is None or t < best[1]):
+                if best is not None:
+                    best[0].close()
+                best = (p, t, sb, mb, learned, wg, is_pick)
+            else:
+                p.close()
     report["learned_within_tol"] = best is not None
     if best is not None:
         report["learned_ms"] = best[1]
         report["sparse_bias"], report["mfma_bias"], report["wg"] = best[2], best[3], best[5]
+        report["router_pick"] = bool(best[6])
         report["learned_routes"] = {int(k): int(v) for k, v in zip(*np.unique(best[4], return_counts=True))}
         if best[1] < t_fixed:
             report["kept"] = "learned"
