@@ -119,8 +119,9 @@ def cpu_baseline_spmm(vbr, routes, X, procs=10):
 def traffic_for(workload, key, stats, **match):
     """HBM bytes per launch of the kept plan from the committed rocprofv3
     FETCH_SIZE + WRITE_SIZE summary (profiles/traffic_<workload>.json), or
-    None unless the profiled entry is the same plan: same key, same
-    algorithmic bytes, and the same options (bias, workgroup size)."""
+    None unless the profiled entry is the same plan: same key (the plan's
+    label, which names its route bias and workgroup size), the same
+    algorithmic bytes and the same workgroup size."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
@@ -212,7 +213,7 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
     kept = "mfma_x6" if "mfma_x6" in res else next(iter(res))
     if good:
         best = min(good, key=lambda k: res[k]["ms"])
-        if res[best]["ms"] < res["mfma_x6"]["ms"]:
+        if "mfma_x6" not in res or res[best]["ms"] < res["mfma_x6"]["ms"]:
             kept = best
     # fresh launches of the kept plan and of the FP32 plain plan (the speedup's denominator)
     for name in dict.fromkeys(k for k in (kept, "fp32_plain") if k in res):
@@ -362,13 +363,29 @@ def bench_batch(args, runtime, info):
     flush = 0 if args.warm else 4 << 30
     log("c5: timing the plain plans launched one by one")
     t_plain = runtime.time_plans(plain, xs, ys_p, warmup=2, reps=args.steps, flush_bytes=flush)
-    t_cands = []
+    def sample_errors(ys):
+        errs = []
+        for i, vbr, x, X in samples:
+            y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                     vbr.bpntrb, vbr.bpntre, x)
+            errs.append(c_oracle.scaled_error(ys[2 * i].numpy(), y64, ay)["componentwise"])
+            Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                     vbr.bpntrb, vbr.bpntre, X)
+            errs.append(c_oracle.scaled_error(ys[2 * i + 1].numpy().ravel(), Y64.ravel(),
+                                              AY.ravel())["componentwise"])
+        return errs
+
+    t_cands, e_cands = [], []
     for j, b in enumerate(biases):
         log(f"c5: timing the batched group at router biases {b}")
         g = runtime.Group(cand[j], xs, ys_l)
         t_cands.append(g.time(warmup=2, reps=args.steps, flush_bytes=flush))
+        g.launch()
+        e_cands.append(max(sample_errors(ys_l)))
         g.close()
-    kb = int(np.argmin([np.mean(t) for t in t_cands]))
+    # the fastest group whose sampled products are within 1e-5 of FP64
+    ok = [j for j in range(len(biases)) if e_cands[j] <= 1e-5] or list(range(len(biases)))
+    kb = min(ok, key=lambda j: np.mean(t_cands[j]))
     learned, bytes_learned, t_batch = cand[kb], cand_bytes[kb], t_cands[kb]
     for j in range(len(biases)):
         if j != kb:
@@ -382,16 +399,19 @@ def bench_batch(args, runtime, info):
     t_learned_unbatched = runtime.time_plans(learned, xs, ys_l, warmup=1, reps=max(3, args.steps // 2),
                                              flush_bytes=flush)
     g.launch()
-    errs = []
-    for i, vbr, x, X in samples:
-        y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
-                                                 vbr.bpntrb, vbr.bpntre, x)
-        errs.append(c_oracle.scaled_error(ys_l[2 * i].numpy(), y64, ay)["componentwise"])
-        Y64, AY = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
-                                                 vbr.bpntrb, vbr.bpntre, X)
-        errs.append(c_oracle.scaled_error(ys_l[2 * i + 1].numpy().ravel(), Y64.ravel(),
-                                          AY.ravel())["componentwise"])
+    errs = sample_errors(ys_l)
     g.close()
+    # the batched SpMV and SpMM launches on their own: groups of only the
+    # SpMV members, then only the SpMM members (plans alternate SpMV, SpMM)
+    per_op = {}
+    for name, sl in (("spmv", slice(0, None, 2)), ("spmm", slice(1, None, 2))):
+        g_op = runtime.Group(learned[sl], xs[sl], ys_l[sl])
+        t_op = float(np.mean(g_op.time(warmup=2, reps=args.steps, flush_bytes=flush)))
+        g_op.close()
+        b_op = int(sum(p.stats["bytes_alg"] for p in learned[sl]))
+        per_op[name] = {"ms": t_op, "bytes_alg": b_op, "achieved_gbs": b_op / (t_op * 1e-3) / 1e9,
+                        "frac_of_hbm_peak": b_op / (t_op * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"c5: batched {name} alone {t_op:.3f} ms, {per_op[name]['achieved_gbs']:.0f} GB/s algorithmic")
     ms = float(np.mean(t_batch))
     ms_plain = float(np.mean(t_plain))
     achieved = bytes_learned / (ms * 1e-3) / 1e9
@@ -415,6 +435,8 @@ def bench_batch(args, runtime, info):
         "speedup_vs_unbatched_plain": ms_plain / ms,
         "bytes_alg_learned": bytes_learned, "bytes_alg_plain": bytes_plain, "flops_alg": flops,
         "sample_errors_vs_fp64": {"max": max(errs), "checked_products": len(errs)},
+        "sample_errors_by_bias": {str(bb): e for bb, e in zip(biases, e_cands)},
+        "batched_by_operation": per_op,
         "device": info,
     }
     print(json.dumps(line), flush=True)
@@ -702,6 +724,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-spmm", action="store_true", help="c2: leave out the 50k SpMM object")
     ap.add_argument("--spmm-variants", default="", help="c3: comma list of the variants to run (profiling)")
+    ap.add_argument("--spmv-plans", default="", help="c2: comma list of the plan labels to run (profiling)")
     ap.add_argument("--cpu-procs", type=int, default=10, help="processes of the CPU timing protocol")
     ap.add_argument("--mode", type=int, default=None, help="default: the autotuned value")
     ap.add_argument("--tile-bytes", type=int, default=None, help="default: the autotuned value")
@@ -784,7 +807,8 @@ def main():
         plans[label] = plan
         return r
 
-    res = {"plain": measure(routes, "plain")}
+    only = [v for v in args.spmv_plans.split(",") if v]      # profiling: a subset of the plans
+    res = {"plain": measure(routes, "plain")} if not only or "plain" in only else {}
     kept = "plain"
     if args.routing in ("learned", "both"):
         from sable_amd.router import Router
@@ -801,6 +825,8 @@ def main():
                                                        vbr.bpntrb, vbr.bpntre, 1, sparse_bias=sb, mfma_bias=mb)
                 for s_i, (lr, wgl) in enumerate(shapes):
                     lab = f"learned sparse_bias={sb:g} wg={wgl}"
+                    if only and lab not in only:
+                        continue
                     r = measure(lr, lab, wg=wgl)
                     r["sparse_bias"] = sb
                     r["wg"] = wgl
@@ -809,13 +835,15 @@ def main():
                     r["label"] = lab
                     cands.append(r)
             good = [r for r in cands if r["error_vs_fp64"]["componentwise"] <= 1e-5]
-            res["learned"] = min(good or cands, key=lambda r: r["ms"])
-            res["learned_candidates"] = [{k: r[k] for k in ("sparse_bias", "wg", "router_pick", "ms", "median_ms",
-                                                            "bytes_alg")} for r in cands]
-            ok = res["learned"]["error_vs_fp64"]["componentwise"] <= 1e-5
-            # keep-if-faster at identical tolerance
-            if args.routing == "learned" or (ok and res["learned"]["ms"] < res["plain"]["ms"]):
-                kept = "learned"
+            if cands:
+                res["learned"] = min(good or cands, key=lambda r: r["ms"])
+                res["learned_candidates"] = [{k: r[k] for k in ("sparse_bias", "wg", "router_pick", "ms",
+                                                                "median_ms", "bytes_alg")} for r in cands]
+                ok = res["learned"]["error_vs_fp64"]["componentwise"] <= 1e-5
+                # keep-if-faster at identical tolerance
+                if (args.routing == "learned" or "plain" not in res
+                        or (ok and res["learned"]["ms"] < res["plain"]["ms"])):
+                    kept = "learned"
         else:
             log("no router weights; learned routing skipped")
     head = res[kept]
@@ -831,8 +859,7 @@ def main():
     st = head["stats"]
     gflops = st["flops_alg"] / (ms * 1e-3) / 1e9
     achieved = st["bytes_alg"] / (ms * 1e-3) / 1e9
-    traffic = traffic_for("spmv_c2" if args.workload == "c2" else "spmv_c1", kept, st,
-                          sparse_bias=head.get("sparse_bias", None), wg=st["wg"])
+    traffic = traffic_for("spmv_c2" if args.workload == "c2" else "spmv_c1", label, st, wg=st["wg"])
 
     cpu_line = None
     if cpu is not None:
@@ -880,7 +907,8 @@ def main():
         "p90_ms": float(np.percentile(times, 90)), "warm_ms": float(np.mean(warm)),
         "error_vs_fp64": errs, "bytes_alg": head["bytes_alg"],
         "routings": res,
-        "learned_speedup_vs_plain": (res["plain"]["ms"] / res["learned"]["ms"]) if "learned" in res else None,
+        "learned_speedup_vs_plain": (res["plain"]["ms"] / res["learned"]["ms"]) if "learned" in res and "plain" in res
+        else None,
         "options": opts,
         "device": info,
     }
