@@ -1,8 +1,8 @@
 """Route selection: the learned MLP router and the keep-if-faster rule.
 
 ``Router.route_and_shape`` predicts the per-block cost of every applicable route
-(dense FP32, sparse, and MFMA when the RHS width allows it and both block
-edges reach 16) under every tile shape, and picks the cheapest.  ``choose_plan`` builds the fixed-threshold plan (the fallback,
+(dense FP32, sparse, and MFMA when the RHS width allows it) under every
+tile shape, and picks the cheapest.  ``choose_plan`` builds the fixed-threshold plan (the fallback,
 threshold 15 as in the reference's mixed flows) and the learned plans, checks
 them against the same FP64 tolerance, times them cold, and keeps the fastest
 learned plan only if it beats the fixed one.
@@ -35,8 +35,12 @@ BIAS_GRID = {"spmv": ((0.0, 0.0), (0.5, 0.0), (1.0, 0.0)),
              "spmm": ((0.0, 0.0), (0.0, 0.5), (0.0, 1.0))}
 
 
-# MFMA tiles are 16 rows by 32 columns: a block goes to MFMA only when both
-# of its edges reach 16 (shorter edges would be mostly padding)
+# MFMA tiles are 16 rows by 32 columns.  The fixed MFMA policy puts a dense
+# block on MFMA only when both of its edges reach 16; the router may put a
+# smaller block there too when its predicted cost (the padding included, as
+# the labels were timed on the padded tiles) is the lowest: on the
+# 256-matrix batch, whose smaller matrices have blocks of a few rows, the
+# edge rule in the router made the SpMM launches 1.5x slower.
 MFMA_MIN_EDGE = 16
 
 
@@ -44,14 +48,12 @@ def mfma_edges_ok(rows, cols):
     return (np.asarray(rows) >= MFMA_MIN_EDGE) & (np.asarray(cols) >= MFMA_MIN_EDGE)
 
 
-def allowed_mask(n_rhs, nblocks, rows=None, cols=None):
+def allowed_mask(n_rhs, nblocks):
     """[nblocks, 3] routes each block may take: MFMA needs N a multiple of 16
-    up to 256 and, when the block edges are given, both edges >= 16."""
+    up to 256."""
     m = np.ones((nblocks, 3), dtype=bool)
     if n_rhs == 1 or n_rhs % 16 or n_rhs > 256:
         m[:, 2] = False
-    elif rows is not None:
-        m[:, 2] = mfma_edges_ok(rows, cols)
     return m
 
 
@@ -87,7 +89,7 @@ class Router:
         pred = self.predict_costs(F).astype(np.float64).reshape(len(F), self.n_shapes, 3)
         pred[:, :, 1] -= float(sparse_bias)
         pred[:, :, 2] -= float(mfma_bias)
-        pred[np.broadcast_to(~allowed_mask(n_rhs, len(F), F[:, 0], F[:, 1])[:, None, :], pred.shape)] = np.inf
+        pred[np.broadcast_to(~allowed_mask(n_rhs, len(F))[:, None, :], pred.shape)] = np.inf
         return pred
 
     def route_and_shape(self, val, indx, bindx, rpntr, cpntr, bpntrb, bpntre, n_rhs=1, sparse_bias=0.0,

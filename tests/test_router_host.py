@@ -1,22 +1,23 @@
-"""Host-side routing rules of the router (no GPU): which routes a block may
-take.  MFMA tiles are 16 x 32, so a block goes to MFMA only when N allows it
-(a multiple of 16 up to 256) and both block edges reach 16."""
+"""Host-side routing rules (no GPU): which routes a block may take.  MFMA
+needs N a multiple of 16 up to 256; the fixed MFMA policy also needs both
+block edges to reach 16 (MFMA tiles are 16 x 32)."""
 import numpy as np
 
 from sable_amd.convert import ROUTE_DENSE, ROUTE_DENSE_MFMA, ROUTE_SPARSE
-from sable_amd.router.router import allowed_mask, fixed_routes
+from sable_amd.router.router import allowed_mask, fixed_routes, mfma_edges_ok
 from sable_amd.vbr import VBR
 
 
-def test_allowed_mask_by_width_and_edges():
-    rows = np.array([8, 16, 32, 15])
-    cols = np.array([32, 16, 8, 200])
-    m = allowed_mask(128, 4, rows, cols)
-    assert m[:, :2].all()
-    assert m[:, 2].tolist() == [False, True, False, False]
+def test_allowed_mask_by_width():
+    for n in (16, 128, 256):
+        assert allowed_mask(n, 4)[:, 2].all()
     for n in (1, 8, 24, 272):
-        assert not allowed_mask(n, 4, rows, cols)[:, 2].any()
-    assert allowed_mask(16, 4)[:, 2].all()          # no edges given: N decides
+        assert not allowed_mask(n, 4)[:, 2].any()
+    assert allowed_mask(1, 4)[:, :2].all()
+
+
+def test_edge_rule():
+    assert mfma_edges_ok([8, 16, 32, 15], [32, 16, 8, 200]).tolist() == [False, True, False, False]
 
 
 def _vbr_two_blocks():
