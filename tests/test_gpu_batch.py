@@ -139,3 +139,27 @@ def test_group_keeps_each_members_split(rt):
     for y, o in zip(ys, own):
         assert np.array_equal(y.numpy(), o)
     g.close()
+
+
+def test_group_of_k_split_members(rt):
+    """SpMM members whose MFMA row tiles are split along k (4 k-steps per
+    chunk): the group's fixup launch adds each member's partial sums, and
+    every output is bit-identical to that plan's own launch over two
+    replays."""
+    mats = _mats(3, seed=19)
+    plans, xs, ys, own = [], [], [], []
+    for vbr, _, X in mats:
+        rm = np.full(len(vbr.bindx), ROUTE_DENSE_MFMA, np.uint8)
+        p = rt.Plan.from_vbr(vbr, rm, op="spmm", n_rhs=16, mfma_kchunk=4, min_tiles=1)
+        plans.append(p)
+        xs.append(rt.DeviceArray.from_numpy(X))
+        ys.append(rt.DeviceArray((p.nrows, 16)))
+        own.append(p.spmm(X))
+    assert any(p.stats["ntiles_split"] > 0 for p in plans)
+    print("case: group of k-split spmm members", flush=True)
+    g = rt.Group(plans, xs, ys)
+    for _ in range(2):
+        g.launch()
+        for y, o in zip(ys, own):
+            assert np.array_equal(y.numpy(), o)
+    g.close()

@@ -204,3 +204,22 @@ def test_small_workgroups_all_routes(rt, n_rhs):
     plan = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs, wg=128)
     assert plan.stats["wg"] == 128
     assert _err(vbr, X, plan.spmm(X))["componentwise"] <= TOL
+
+
+@pytest.mark.parametrize("kchunk", [1, 4, -1])
+def test_kchunk_split_tiles(rt, kchunk):
+    """MFMA row tiles split along k (mfma_kchunk k-steps per chunk, partial
+    sums added by the fixup launch in chunk order): within 1e-5 of FP64 and
+    the same bits on every launch; -1 splits nothing."""
+    vbr = _c3_like(6000, seed=17)
+    X = np.random.default_rng(18).uniform(-1, 1, (6000, 128)).astype(np.float32)
+    routes = np.full(len(vbr.bindx), ROUTE_DENSE_MFMA, np.uint8)
+    print(f"case: mfma k-split chunk {kchunk}", flush=True)
+    p = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=128, mfma_kchunk=kchunk)
+    st = p.stats
+    assert (st["ntiles_split"] > 0) == (kchunk > 0)
+    assert (st["scratch_bytes"] > 0) == (kchunk > 0)
+    Y1 = p.spmm(X)
+    Y2 = p.spmm(X)
+    assert np.array_equal(Y1, Y2)
+    assert _err(vbr, X, Y1)["componentwise"] <= TOL
