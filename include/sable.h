@@ -94,7 +94,7 @@ typedef struct {
                                  k-steps per work item; a longer row tile is split
                                  along k into chunks whose partial sums a fixup
                                  launch adds in chunk order (deterministic);
-                                 0 = 32, negative = no split */
+                                 0 or negative = no split */
 } SablePlanOpts;
 
 typedef struct {

@@ -47,6 +47,13 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 #ifndef SB_MFMA_XDEPTH
 #define SB_MFMA_XDEPTH 2
 #endif
+// the compiler's scheduling strategy for a k-step's basic block
+// (__builtin_amdgcn_iglp_opt), -1: none.  Strategy 3 ran the 50k SpMM at
+// N = 128 6 % faster than none (116-117 vs 124 us, bit-identical) and the
+// 256-matrix batch at N = 16 the same
+#ifndef SB_MFMA_IGLP
+#define SB_MFMA_IGLP 3
+#endif
 // 1: the issue order of a k-step is set explicitly (MFMAs interleaved with
 // the splits' VALU work); 0: the compiler's schedule
 #ifndef SB_MFMA_INTERLEAVE
@@ -371,7 +378,7 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
             constexpr int nsl = (sl + 1) % D;
             split_a(ar[nsl]);
             load_a(ar[nsl], ks + 1 + D < K ? ks + 1 + D : K - 1);
-#if defined(SB_MFMA_IGLP) && SB_MFMA_IGLP >= 0
+#if SB_MFMA_IGLP >= 0
             __builtin_amdgcn_iglp_opt(SB_MFMA_IGLP);
 #endif
         }
