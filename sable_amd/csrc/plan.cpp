@@ -389,13 +389,16 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     S.n_rhs = P.n_rhs;
     S.nblocks = v.nblocks;
 
-    // rows per tile bound: SpMV needs hs <= wg (flat stream lanes); the FP32
-    // SpMM kernel needs hs <= (wg / (N/4)) * 8 register rows.
+    // rows per tile bound: SpMV needs hs <= wg (flat stream lanes).  The FP32
+    // SpMM kernel's wide shape takes 4 rows per thread group, (wg / (N/8)) * 4
+    // rows (8 rows per group measured slower: twice the registers and half
+    // the groups busy); its narrow shape at most 8, (wg / (N/4)) * 8 rows and
+    // 64.  MFMA tiles are cut separately: multiples of 16 rows, at most
+    // mfma_max_rows.
     int max_rows = std::min(o.max_seg_rows, o.wg);
-    // SpMM: the FP32 kernel holds at most 8 rows per thread group
-    // ((wg / (N/4)) * 8 rows, and 64 rows); MFMA tiles are cut separately:
-    // multiples of 16 rows, at most mfma_max_rows.
-    if (spmm) max_rows = std::min({max_rows, (o.wg / (o.n_rhs / 4)) * 8, 64});
+    if (spmm)
+        max_rows = SB_F32_WIDE(o.n_rhs) ? std::min(max_rows, (o.wg / (o.n_rhs / 8)) * SB_F32_WIDE_ROWS)
+                                        : std::min({max_rows, (o.wg / (o.n_rhs / 4)) * 8, 64});
     const int mfma_rows = spmm && o.n_rhs % 16 == 0
                               ? std::max(16, std::min(SB_MFMA_MAX_ROWS(o.n_rhs, o.wg), (o.max_seg_rows / 16) * 16))
                               : 16;
@@ -532,6 +535,8 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                 SbPiece pc{};
                 pc.kind = SB_PIECE_DENSE;
                 pc.ncols = group_width(g);
+                // SpMM: columns padded to a multiple of 4 rows (float4 row quads)
+                pc.pitch = spmm ? (hs + 3) / 4 * 4 : hs;
                 while (P.dval.size() % 4) P.dval.push_back(0.f);
                 pc.data_off = (int64_t)P.dval.size();
                 for (auto &cs : g) {
@@ -539,6 +544,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                     for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c) {
                         const float *col = bv + (int64_t)c * h + s0;
                         P.dval.insert(P.dval.end(), col, col + hs);
+                        P.dval.insert(P.dval.end(), pc.pitch - hs, 0.f);
                     }
                 }
                 while (P.dval.size() % 4) P.dval.push_back(0.f);

@@ -306,7 +306,7 @@ const char *kStubSpmv =
     "extern \"C\" __global__ void __launch_bounds__(SB_WG, SB_SPMV_WAVES(SB_UNROLL)) sb_spmv_rtc(SbSpmvArgs a)\n"
     "{ sb_spmv_body<SB_WG, SB_UNROLL, SB_XCAP>(a); }\n";
 const char *kStubSpmmF32 =
-    "extern \"C\" __global__ void __launch_bounds__(SB_WG) sb_spmm_f32_rtc(SbSpmmArgs a)\n"
+    "extern \"C\" __global__ void __launch_bounds__(SB_WG, SB_F32_WAVES(SB_NRHS)) sb_spmm_f32_rtc(SbSpmmArgs a)\n"
     "{ sb_spmm_f32_body<SB_WG, SB_RTMAX, SB_NRHS, SB_XCAP>(a); }\n";
 const char *kStubSpmmMfma =
     "extern \"C\" __global__ void __launch_bounds__(SB_WG, SB_MFMA_WAVES) sb_spmm_mfma_rtc(SbSpmmArgs a)\n"
@@ -448,13 +448,19 @@ int select_phase(const sable::HostPlan &h, const SablePlanOpts &o, Phase &ph)
     }
     // SpMV: room for the largest staging group the host formed (at most
     // SB_STAGE_CAP + 4 slots, so the buffer never exceeds 4096 + 4)
-    s.xcap = spmm ? std::min(4096, pow2_at_least(maxcols + 1))
+    // SpMM: the widest piece's staged columns (the MFMA kernel's k-steps
+    // read up to 31 past them in its padded xrow; the FP32 kernel's X
+    // staging buffer is sized by it, so narrower pieces use less LDS)
+    s.xcap = spmm ? std::min(4096, pow2_at_least(mfma ? maxcols + 1 : maxcols))
                   : std::min(sable::SB_STAGE_CAP, std::max(16, pow2_at_least(maxstage)));
     s.split = o.mfma_split == 2 ? 2 : 3;
     if (spmm) {
         s.nrhs = h.n_rhs;
-        const int ngroups = std::max(1, o.wg / (h.n_rhs / 4));
-        s.rtmax = mfma ? 8 : std::max(1, (rows + ngroups - 1) / ngroups);
+        // FP32 kernel: rows per thread group (the wide shape's are whole row quads)
+        const bool wide = SB_F32_WIDE(h.n_rhs);
+        const int ngroups = std::max(1, o.wg / (h.n_rhs / SB_F32_COLS(h.n_rhs)));
+        const int rt = std::max(1, (rows + ngroups - 1) / ngroups);
+        s.rtmax = mfma ? 8 : wide ? (rt + 3) / 4 * 4 : rt;
     }
     if (spmm)
         ph.aot = mfma ? (s.split == 2 ? (const void *)sb_spmm_mfma2_aot : (const void *)sb_spmm_mfma_aot)

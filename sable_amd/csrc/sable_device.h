@@ -58,6 +58,17 @@ struct SbTile {
 // gives such a piece at most SB_SPMM_XS_FLOATS / N columns.
 #define SB_SPMM_XS_FLOATS 16384
 
+// FP32 SpMM thread shape.  Wide (N a multiple of 8, at least 64): a thread
+// holds SB_F32_WIDE_ROWS rows (4 or 8) x 8 columns (two float4 column quads,
+// one in each half of N), so a tile has at most (WG / (N/8)) *
+// SB_F32_WIDE_ROWS rows.  Otherwise: up to 8 rows x one float4 column quad,
+// at most (WG / (N/4)) * 8 rows and 64.
+#define SB_F32_WIDE(N) ((N) >= 64 && (N) % 8 == 0)
+#define SB_F32_COLS(N) (SB_F32_WIDE(N) ? 8 : 4)
+#ifndef SB_F32_WIDE_ROWS
+#define SB_F32_WIDE_ROWS 4
+#endif
+
 // SpMM MFMA tiles: a wavefront holds SB_MFMA_UNITS(N) 16-row tiles across
 // all N/16 column tiles (their FP32 accumulators plus the double-buffered A
 // fragments), which bounds a tile at SB_MFMA_MAX_ROWS(N, WG) rows (and 256).
@@ -86,7 +97,9 @@ struct SbPiece {
     int32_t xmap_off;        // xmap[xmap_off .. + ncols + 4): x column of each staged
                              // slot, -1 for the zero pads; consecutive pieces of a
                              // tile are consecutive in xmap
-    int32_t reserved;
+    int32_t pitch;           // dense: floats from one panel column to the next (SpMV:
+                             // the tile's rows; SpMM: the rows rounded up to 4, the
+                             // extra rows zero, so a column's row quads are float4s)
 };
 
 // A contiguous run of x (or X rows) copied into the staging buffer.

@@ -450,11 +450,13 @@ __device__ __forceinline__ void sb_spmv_body(const SbSpmvArgs &a)
 // ---------------------------------------------------------------------------
 // SpMM, FP32 on the vector ALUs (the plain path)
 //
-// Thread (rg, v): row group rg = tid / NV owns rows rg*rt .. rg*rt+rt-1 of the
-// tile and RHS columns 4v..4v+3 (NV = n_rhs/4).  Dense piece values are read
-// once per (row, column) and broadcast to the NV lanes of a row group; X rows
-// come through L2 (each row is shared by the row groups of the tile).  Sparse
-// rows are walked entry by entry with the same thread mapping.
+// Narrow shape (N not SB_F32_WIDE): thread (rg, v): row group rg = tid / NV
+// owns rows rg*rt .. rg*rt+rt-1 of the tile and RHS columns 4v..4v+3 (NV =
+// n_rhs/4).  Dense piece values are read once per (row, column) and
+// broadcast to the NV lanes of a row group.  The wide shape (below) holds 8
+// columns per thread and reads A as float4 row quads.  Both stage a piece's X
+// rows in LDS once per tile and walk sparse rows entry by entry with the same
+// thread mapping.
 
 // Expands a piece's x ranges into xrow[0..ncols): the X row of every column.
 __device__ __forceinline__ void sb_expand_xrow(const SbXRange *__restrict__ xr, int r0, int r1, int *xrow)
@@ -549,7 +551,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPi
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 xv[u] = xs4[(c + u) * NV + m.v];
-                const SB_G float *__restrict__ col = P + (size_t)(c + u) * hs;
+                const SB_G float *__restrict__ col = P + (size_t)(c + u) * pc.pitch;
 #pragma unroll
                 for (int i = 0; i < RTMAX; ++i) {
                     const int r = m.rbeg + i;
@@ -563,7 +565,7 @@ __device__ __forceinline__ void sb_spmm_vec_piece(const SbMatBufs &a, const SbPi
         }
         for (; c < pc.ncols; ++c) {
             const float4 xv = xs4[c * NV + m.v];
-            const SB_G float *__restrict__ col = P + (size_t)c * hs;
+            const SB_G float *__restrict__ col = P + (size_t)c * pc.pitch;
 #pragma unroll
             for (int i = 0; i < RTMAX; ++i) {
                 const int r = m.rbeg + i;
@@ -653,6 +655,290 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
     }
 }
 
+// Wide shape (SB_F32_WIDE(N)): thread (rg, v) of NV8 = N/8 lanes per row
+// group holds rows [rbeg, rbeg + rt) (rt a multiple of 4) and the column
+// quads 4v and N/2 + 4v, so the lanes of a row group read consecutive
+// float4s of the staged X row in each half.  A dense column's row quads are
+// float4 loads from the padded panel (pitch: the rows rounded up to 4),
+// requested one column pair ahead: two vector-memory loads and four LDS
+// loads per 64 FMAs at rt = 8.
+struct SbWideMap {
+    int rt;       // rows per row group (multiple of 4, uniform over the tile)
+    int rbeg;     // first tile row of this thread's group
+    int v;        // lane in the row group
+    int nq;       // row quads of this thread inside the panel
+    bool active;
+};
+
+// quads: rt rounded up to whole row quads (tiles with dense pieces, whose A
+// columns are read as float4 row quads); else rt = ceil(hs / groups)
+__device__ __forceinline__ SbWideMap sb_wide_map(int hs, int N, int wg, bool quads)
+{
+    SbWideMap m;
+    const int NV8 = N >> 3;
+    const int ngroups = wg / NV8;
+    m.rt = (hs + ngroups - 1) / ngroups;
+    if (quads) m.rt = (m.rt + 3) & ~3;
+    const int rg = threadIdx.x / NV8;
+    m.v = threadIdx.x - rg * NV8;
+    m.rbeg = rg * m.rt;
+    m.active = rg < ngroups && m.rbeg < hs;
+    const int pitch = (hs + 3) & ~3;
+    m.nq = m.active ? min(m.rt, pitch - m.rbeg) >> 2 : 0;
+    return m;
+}
+
+// Wide FP32 SpMM tuning: column pairs of A in flight per thread (register
+// sets of the ring), sparse entries per unrolled step, and the waves per
+// SIMD the launch bound asks of the compiler (4: at most 128 VGPRs, two
+// 512-thread workgroups per CU, which the X staging LDS also allows)
+#ifndef SB_F32_DEPTH
+#define SB_F32_DEPTH 4
+#endif
+#ifndef SB_F32_SPUNROLL
+#define SB_F32_SPUNROLL 4
+#endif
+#ifndef SB_F32_SPCHUNK
+#define SB_F32_SPCHUNK 1536
+#endif
+#ifndef SB_F32_MINWAVES
+#define SB_F32_MINWAVES 4
+#endif
+#define SB_F32_WAVES(N) (SB_F32_WIDE(N) ? SB_F32_MINWAVES : 1)
+
+// r[j] += v * x[j] over the thread's 8 columns
+__device__ __forceinline__ void sb_wide_fma1(float (&r)[8], float v, const float4 &x0, const float4 &x1)
+{
+    r[0] += v * x0.x;
+    r[1] += v * x0.y;
+    r[2] += v * x0.z;
+    r[3] += v * x0.w;
+    r[4] += v * x1.x;
+    r[5] += v * x1.y;
+    r[6] += v * x1.z;
+    r[7] += v * x1.w;
+}
+
+// acc[4q + k][j] += a[q][k] * x[j]
+template <int RQ>
+__device__ __forceinline__ void sb_wide_fma(float (&acc)[4 * RQ][8], const sb_f4 (&a)[RQ], const float4 &x0,
+                                            const float4 &x1)
+{
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sb_wide_fma1(acc[4 * q + k], a[q][k], x0, x1);
+    }
+}
+
+// The thread's row quads of panel column c.  Quads past the panel (only in
+// the last row group) repeat its last quad: their rows are never stored.
+template <int RQ>
+__device__ __forceinline__ void sb_wide_load(const SB_G float *__restrict__ P, int pitch, int c, int nq,
+                                             sb_f4 (&a)[RQ])
+{
+    const SB_G sb_f4 *__restrict__ col = reinterpret_cast<const SB_G sb_f4 *>(P + (size_t)c * pitch);
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) a[q] = col[min(q, nq - 1)];
+}
+
+// One dense piece: acc += panel x staged X rows.
+template <int RQ>
+__device__ __forceinline__ void sb_spmm_wide_dense(const SbMatBufs &a, const SbPiece &pc, int N, const SbWideMap &m,
+                                                   const float4 *xs4, float (&acc)[4 * RQ][8])
+{
+    const int NV = N >> 2, H = N >> 3;
+    {
+        if (!m.active) return;
+        // column pairs through a ring of SB_F32_DEPTH register sets: the
+        // set just consumed is refilled with the pair SB_F32_DEPTH ahead
+        // (clamped to the last column, so always in bounds)
+        constexpr int D = SB_F32_DEPTH;
+        const int n = pc.ncols, pitch = pc.pitch, np = n >> 1;
+        const SB_G float *__restrict__ P = sb_gp(a.dval) + pc.data_off + m.rbeg;
+        sb_f4 buf[D][2][RQ];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            sb_wide_load<RQ>(P, pitch, min(2 * d, n - 1), m.nq, buf[d][0]);
+            sb_wide_load<RQ>(P, pitch, min(2 * d + 1, n - 1), m.nq, buf[d][1]);
+        }
+        int p = 0;
+        for (; p + D <= np; p += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int c = 2 * (p + d);
+                const float4 x00 = xs4[c * NV + m.v], x01 = xs4[c * NV + H + m.v];
+                const float4 x10 = xs4[(c + 1) * NV + m.v], x11 = xs4[(c + 1) * NV + H + m.v];
+                sb_wide_fma<RQ>(acc, buf[d][0], x00, x01);
+                sb_wide_fma<RQ>(acc, buf[d][1], x10, x11);
+                sb_wide_load<RQ>(P, pitch, min(c + 2 * D, n - 1), m.nq, buf[d][0]);
+                sb_wide_load<RQ>(P, pitch, min(c + 2 * D + 1, n - 1), m.nq, buf[d][1]);
+                // keep the refill ahead of the next set's FMAs (the
+                // scheduler otherwise sinks it to the end of the loop)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // the last (np - p) < D pairs are in buf[0 ..]
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d) {
+            if (p + d < np) {
+                const int c = 2 * (p + d);
+                sb_wide_fma<RQ>(acc, buf[d][0], xs4[c * NV + m.v], xs4[c * NV + H + m.v]);
+                sb_wide_fma<RQ>(acc, buf[d][1], xs4[(c + 1) * NV + m.v], xs4[(c + 1) * NV + H + m.v]);
+            }
+        }
+        if (n & 1) {    // odd column count: the last column on its own
+            sb_f4 l[RQ];
+            sb_wide_load<RQ>(P, pitch, n - 1, m.nq, l);
+            sb_wide_fma<RQ>(acc, l, xs4[(n - 1) * NV + m.v], xs4[(n - 1) * NV + H + m.v]);
+        }
+        return;
+    }
+}
+
+// One sparse piece (CSR rows of the tile): its entries are staged in LDS in
+// chunks of SPCH (value, local column), then each thread walks its rows'
+// entries, U entries' loads in flight at a time.
+template <int WG, int RQ, int SPCH, int U>
+__device__ __forceinline__ void sb_spmm_wide_sparse(const SbMatBufs &a, const SbPiece &pc, int hs, int N,
+                                                    const SbWideMap &m, const float4 *xs4,
+                                                    float (&acc)[4 * RQ][8], float *spv, int *spx)
+{
+    const int NV = N >> 2, H = N >> 3;
+    const SB_G int32_t *__restrict__ rp = sb_gp(a.srowptr) + pc.rowptr_off;
+    const SB_G float *__restrict__ sv = sb_gp(a.sval) + pc.data_off;
+    const SB_G uint16_t *__restrict__ si = sb_gp(a.sidx) + pc.data_off;
+    const int nnzp = rp[hs];
+    int rlo[4 * RQ], rhi[4 * RQ];
+#pragma unroll
+    for (int i = 0; i < 4 * RQ; ++i) {
+        const int r = m.rbeg + i;
+        const bool ok = m.active && i < m.rt && r < hs;
+        rlo[i] = ok ? rp[r] : 0;
+        rhi[i] = ok ? rp[r + 1] : 0;
+    }
+    for (int e0 = 0; e0 < nnzp; e0 += SPCH) {
+        const int cnt = min(SPCH, nnzp - e0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt; e += WG) {
+            spv[e] = sv[e0 + e];
+            spx[e] = si[e0 + e];
+        }
+        __syncthreads();
+        if (!m.active) continue;
+#pragma unroll
+        for (int i = 0; i < 4 * RQ; ++i) {
+            const int lo = max(rlo[i], e0) - e0, hi = min(rhi[i], e0 + cnt) - e0;
+            // SB_F32_SPUNROLL entries' loads in flight, then their FMAs in
+            // entry order
+            int e = lo;
+            for (; e + U <= hi; e += U) {
+                float v[U];
+                float4 x0[U], x1[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    v[u] = spv[e + u];
+                    const int ci = spx[e + u];
+                    x0[u] = xs4[ci * NV + m.v];
+                    x1[u] = xs4[ci * NV + H + m.v];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) sb_wide_fma1(acc[i], v[u], x0[u], x1[u]);
+            }
+            for (; e < hi; ++e) {
+                const int ci = spx[e];
+                sb_wide_fma1(acc[i], spv[e], xs4[ci * NV + m.v], xs4[ci * NV + H + m.v]);
+            }
+        }
+    }
+}
+
+// Stages the X rows of piece pc into xs4 (after every thread is done with
+// the previous piece's).
+template <int WG>
+__device__ __forceinline__ void sb_wide_stage(const SbSpmmArgs &a, const SbPiece &pc, const SB_G float *X, int N,
+                                              int *xrow, float4 *xs4)
+{
+    __syncthreads();
+    sb_expand_xrow(a.xr, pc.xr_begin, pc.xr_end, xrow);
+    __syncthreads();
+    sb_stage_xrows<WG>(X, N, xrow, pc.ncols, xs4);
+    __syncthreads();
+}
+
+// One tile with up to 4 * RQ rows per thread group: its dense pieces first,
+// then its sparse pieces (two loops, so the registers the dense loop keeps
+// are free in the sparse one).  DENSE false: the tile has no dense pieces.
+template <int WG, int RQ, int NRHS, bool DENSE>
+__device__ __forceinline__ void sb_spmm_wide_tile_q(const SbSpmmArgs &a, const SbTile &tile, const SbWideMap &m,
+                                                    int *xrow, float4 *xs4, float *spv, int *spx)
+{
+    const int N = NRHS ? NRHS : a.n_rhs;
+    const int hs = tile.nrows;
+    const SbMatBufs mb = sb_mat_bufs(a, tile.mat, nullptr);
+    const SB_G float *__restrict__ X = sb_gp(mb.x);
+    SB_G float *__restrict__ Y = sb_gp(mb.y);
+    float acc[4 * RQ][8];
+#pragma unroll
+    for (int i = 0; i < 4 * RQ; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.f;
+
+    if constexpr (DENSE) {
+        for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
+            const SbPiece pc = a.pieces[pi];
+            if (pc.kind != SB_PIECE_DENSE) continue;
+            sb_wide_stage<WG>(a, pc, X, N, xrow, xs4);
+            sb_spmm_wide_dense<RQ>(mb, pc, N, m, xs4, acc);
+        }
+    }
+    for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) {
+        const SbPiece pc = a.pieces[pi];
+        if (pc.kind != SB_PIECE_SPARSE) continue;
+        sb_wide_stage<WG>(a, pc, X, N, xrow, xs4);
+        sb_spmm_wide_sparse<WG, RQ, SB_F32_SPCHUNK, SB_F32_SPUNROLL>(mb, pc, hs, N, m, xs4, acc, spv, spx);
+    }
+    if (!m.active) return;
+#pragma unroll
+    for (int i = 0; i < 4 * RQ; ++i) {
+        const int r = m.rbeg + i;
+        if (i < m.rt && r < hs) {
+            SB_G float *row = Y + (size_t)(tile.row0 + r) * N + 4 * m.v;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                SB_G sb_f4 *dst = reinterpret_cast<SB_G sb_f4 *>(row + h * (N >> 1));
+                sb_f4 o = {acc[i][4 * h], acc[i][4 * h + 1], acc[i][4 * h + 2], acc[i][4 * h + 3]};
+                if (tile.y_accum) {      // the rows' MFMA tile wrote Y in the previous launch
+                    const sb_f4 w = *dst;
+                    o = sb_f4{w.x + o.x, w.y + o.y, w.z + o.z, w.w + o.w};
+                }
+                *dst = o;
+            }
+        }
+    }
+}
+
+// rt (uniform over the tile, at most 4 * RQ) picks the instance: 4 or 8
+// rows per thread group, with or without the dense loop.
+template <int WG, int RQ, int NRHS>
+__device__ __forceinline__ void sb_spmm_wide_tile(const SbSpmmArgs &a, int t, int *xrow, float4 *xs4, float *spv,
+                                                  int *spx)
+{
+    static_assert(RQ == 1 || RQ == 2, "the wide shape holds 4 or 8 rows per thread group");
+    const int N = NRHS ? NRHS : a.n_rhs;
+    const SbTile tile = a.tiles[t];
+    bool dense = false;
+    for (int pi = tile.piece_begin; pi < tile.piece_end; ++pi) dense = dense || a.pieces[pi].kind == SB_PIECE_DENSE;
+    const SbWideMap m = sb_wide_map(tile.nrows, N, WG, dense);
+    if (dense) {
+        if (RQ == 2 && m.rt > 4) sb_spmm_wide_tile_q<WG, RQ, NRHS, true>(a, tile, m, xrow, xs4, spv, spx);
+        else sb_spmm_wide_tile_q<WG, 1, NRHS, true>(a, tile, m, xrow, xs4, spv, spx);
+    } else {
+        if (RQ == 2 && m.rt > 4) sb_spmm_wide_tile_q<WG, RQ, NRHS, false>(a, tile, m, xrow, xs4, spv, spx);
+        else sb_spmm_wide_tile_q<WG, 1, NRHS, false>(a, tile, m, xrow, xs4, spv, spx);
+    }
+}
+
 template <int WG, int RTMAX, int NRHS, int XCAP>
 __device__ __forceinline__ void sb_spmm_f32_body(const SbSpmmArgs &a)
 {
@@ -662,9 +948,27 @@ __device__ __forceinline__ void sb_spmm_f32_body(const SbSpmmArgs &a)
                              : SB_SPMM_XS_FLOATS / 4;
     __shared__ int xrow[XCAP + 4];
     __shared__ float4 xs4[XS4];
-    __shared__ float spv[SB_SPMM_SPCHUNK];
-    __shared__ int spx[SB_SPMM_SPCHUNK];
+    // sparse entry chunks: SB_F32_SPCHUNK entries in the wide shape,
+    // SB_SPMM_SPCHUNK in the narrow one (the generic instance runs either)
+    constexpr int SPC = NRHS == 0 ? (SB_F32_SPCHUNK > SB_SPMM_SPCHUNK ? SB_F32_SPCHUNK : SB_SPMM_SPCHUNK)
+                        : SB_F32_WIDE(NRHS) ? SB_F32_SPCHUNK : SB_SPMM_SPCHUNK;
+    __shared__ float spv[SPC];
+    __shared__ int spx[SPC];
     __shared__ int next_tile;
-    sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
-                 [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow, xs4, spv, spx); });
+    // wide: tiles hold at most SB_F32_WIDE_ROWS rows per thread group (the
+    // plan builder cuts them so); RTMAX <= 4 leaves out the 8-row instance
+    constexpr int RQ = RTMAX > 4 ? 2 : 1;
+    if constexpr (NRHS != 0 && SB_F32_WIDE(NRHS)) {
+        sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
+                     [&](int t) { sb_spmm_wide_tile<WG, RQ, NRHS>(a, t, xrow, xs4, spv, spx); });
+    } else if constexpr (NRHS != 0) {
+        sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
+                     [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow, xs4, spv, spx); });
+    } else if (SB_F32_WIDE(a.n_rhs)) {     // generic instance: the shape follows N
+        sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
+                     [&](int t) { sb_spmm_wide_tile<WG, SB_F32_WIDE_ROWS / 4, 0>(a, t, xrow, xs4, spv, spx); });
+    } else {
+        sb_for_tiles(a.ntiles, a.mode, a.queue, &next_tile,
+                     [&](int t) { sb_spmm_f32_tile<WG, RTMAX, NRHS>(a, t, xrow, xs4, spv, spx); });
+    }
 }

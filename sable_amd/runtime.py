@@ -224,7 +224,7 @@ class Plan:
     PIECE_DTYPE = np.dtype([("data_off", "<i8"), ("kind", "<i4"), ("ncols", "<i4"),
                             ("xr_begin", "<i4"), ("xr_end", "<i4"), ("rowptr_off", "<i4"),
                             ("ksteps", "<i4"), ("grp_end", "<i4"), ("stage_len", "<i4"),
-                            ("xmap_off", "<i4"), ("reserved", "<i4")])
+                            ("xmap_off", "<i4"), ("pitch", "<i4")])
 
     def export(self, name):
         which, dt = self._EXPORT[name]

@@ -106,7 +106,7 @@ def test_mfma_short_and_unit_blocks(rt):
     assert _err(vbr, X, Y)["componentwise"] <= TOL
 
 
-@pytest.mark.parametrize("n_rhs", [16, 48])
+@pytest.mark.parametrize("n_rhs", [16, 48, 128])
 def test_spmm_edge_shapes(rt, n_rhs):
     """Unit partitions and empty block rows on every route; one 700 x 5000
     block (wider than the FP32 path's LDS staging and the MFMA staging
@@ -150,6 +150,26 @@ def test_matches_reference_spmm_program(rt):
         print(f"case: vs reference program route={r}", flush=True)
         Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=16).spmm(X)
         assert c_oracle.scaled_error(Y.ravel(), Yc.astype(np.float64).ravel(), AY.ravel())["componentwise"] <= 2 * TOL
+
+
+@pytest.mark.parametrize("n_rhs", [16, 128])
+def test_fp32_generic_kernel(rt, n_rhs):
+    """The ahead-of-time FP32 kernel (no hipRTC, N read at run time) takes
+    the thread shape of the specialised one (wide at N = 128, narrow at
+    N = 16) on the same tiles; both within the tolerance of FP64, with
+    dense and sparse pieces in the same tiles and sparse-only tiles after
+    MFMA tiles."""
+    vbr = _c3_like(2000, seed=31)
+    rng = np.random.default_rng(32)
+    X = rng.uniform(-1, 1, (vbr.cpntr[-1], n_rhs)).astype(np.float32)
+    thr = threshold_routes(vbr.val, vbr.indx, 15)
+    mixed = np.where(thr == ROUTE_DENSE, ROUTE_DENSE_MFMA, ROUTE_SPARSE).astype(np.uint8)
+    for name, routes in (("fp32", thr), ("mfma+sparse", mixed)):
+        for use_rtc in (True, False):
+            print(f"case: generic kernel n={n_rhs} {name} rtc={use_rtc}", flush=True)
+            plan = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs, wg=256, use_rtc=use_rtc)
+            assert plan.stats["rtc_used"] == int(use_rtc)
+            assert _err(vbr, X, plan.spmm(X))["componentwise"] <= TOL
 
 
 def test_spmm_device_arrays_and_modes(rt):

@@ -96,8 +96,12 @@ def reconstruct(plan):
             off = int(p["data_off"])
             if p["kind"] == 0:
                 assert off % 4 == 0
-                blk = dval[off:off + hs * len(cols)].reshape(len(cols), hs).T
-                M[r0:r0 + hs][:, cols] += blk
+                pitch = int(p["pitch"])
+                # SpMM panels pad each column to a multiple of 4 rows with zeros
+                assert pitch == ((hs + 3) // 4 * 4 if plan.op == "spmm" else hs)
+                pan = dval[off:off + pitch * len(cols)].reshape(len(cols), pitch)
+                assert np.all(pan[:, hs:] == 0)
+                M[r0:r0 + hs][:, cols] += pan[:, :hs].T
             elif p["kind"] == 1 and plan.op == "spmv":
                 for r, (lc, v) in slice_rows(t, p, srow, sval, sidx).items():
                     assert np.all(np.diff(lc) > 0)  # ascending columns in a row
