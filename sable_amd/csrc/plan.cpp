@@ -171,9 +171,13 @@ void stage_groups(HostPlan &P, const SbTile &t)
     }
 }
 
-// Nonzeros of rows [s0, s0+hs) of a block row (height h) restricted to one
-// column group, row by row in column order (local column = position in the
-// group's concatenated columns).  Returns false when there are none.
+// Nonzeros of rows [s0, s0+hs) of a row span restricted to one column group,
+// row by row in column order (local column = position in the group's
+// concatenated columns).  The span is one block row, or a run of short block
+// rows planned as one (SpMV): geo gives each block's first row in the span
+// and its height.  keep[k]: the block's zeros are entries too (a dense-routed
+// block of such a run, so its products are the dense path's).  Returns false
+// when there are none.
 struct SparseRows {
     int32_t group = 0;
     std::vector<int32_t> rp;      // hs + 1 row pointers
@@ -181,15 +185,36 @@ struct SparseRows {
     std::vector<uint16_t> col;
 };
 
-bool collect_sparse_rows(const SableVbr &v, const std::vector<ColSeg> &g, int32_t h, int32_t s0, int32_t hs,
+struct SpanGeo {
+    const SableVbr *v;
+    const int32_t *brow;          // block row of every block
+    int32_t r0;                   // first row of the span
+    const std::vector<char> *keep;
+    int32_t off(int64_t k) const { return v->rpntr[brow[k]] - r0; }
+    int32_t height(int64_t k) const { return v->rpntr[brow[k] + 1] - v->rpntr[brow[k]]; }
+    bool keeps(int64_t k) const { return keep && (*keep)[k]; }
+};
+
+bool collect_sparse_rows(const SableVbr &v, const std::vector<ColSeg> &g, const SpanGeo &geo, int32_t s0, int32_t hs,
                          SparseRows &out)
 {
+    // tile rows [s0, s0+hs) that block k covers: [lo, hi), block row br = r + s0 - off
+    auto span = [&](int64_t k, int32_t &lo, int32_t &hi, int32_t &shift) {
+        const int32_t off = geo.off(k), bh = geo.height(k);
+        lo = std::max(0, off - s0);
+        hi = std::min(hs, off + bh - s0);
+        shift = s0 - off;
+    };
     std::vector<int32_t> cnt(hs, 0);
     for (auto &cs : g) {
         const float *bv = v.val + v.indx[cs.k];
+        const int32_t bh = geo.height(cs.k);
+        const bool keep = geo.keeps(cs.k);
+        int32_t lo, hi, shift;
+        span(cs.k, lo, hi, shift);
         for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c) {
-            const float *col = bv + (int64_t)c * h + s0;
-            for (int32_t r = 0; r < hs; ++r) cnt[r] += (col[r] != 0.0f);
+            const float *col = bv + (int64_t)c * bh + shift;
+            for (int32_t r = lo; r < hi; ++r) cnt[r] += (keep || col[r] != 0.0f);
         }
     }
     out.rp.assign(hs + 1, 0);
@@ -202,10 +227,14 @@ bool collect_sparse_rows(const SableVbr &v, const std::vector<ColSeg> &g, int32_
     int32_t lc = 0;
     for (auto &cs : g) {
         const float *bv = v.val + v.indx[cs.k];
+        const int32_t bh = geo.height(cs.k);
+        const bool keep = geo.keeps(cs.k);
+        int32_t lo, hi, shift;
+        span(cs.k, lo, hi, shift);
         for (int32_t c = cs.cb; c < cs.cb + cs.n; ++c, ++lc) {
-            const float *col = bv + (int64_t)c * h + s0;
-            for (int32_t r = 0; r < hs; ++r) {
-                if (col[r] != 0.0f) {
+            const float *col = bv + (int64_t)c * bh + shift;
+            for (int32_t r = lo; r < hi; ++r) {
+                if (keep || col[r] != 0.0f) {
                     out.val[cur[r]] = col[r];
                     out.col[cur[r]] = (uint16_t)lc;
                     ++cur[r];
@@ -408,11 +437,47 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     // count nonzeros (plan-invariant flops basis)
     for (int64_t i = 0; i < v.indx[v.nblocks]; ++i) S.nnz += (v.val[i] != 0.0f);
 
-    for (int32_t a = 0; a < v.nbr; ++a) {
-        const int32_t r0 = v.rpntr[a], h = v.rpntr[a + 1] - r0;
+    // SpMV: a run of consecutive short block rows (at most SB_SPMV_SHORT_ROWS
+    // rows and SB_SPMV_SHORT_VALUES stored values each) is planned as one row
+    // span of at most SB_SPMV_SPAN_ROWS rows and SB_SPMV_SPAN_VALUES stored
+    // values: one tile in which every block takes the sparse path (a
+    // dense-routed block's zeros included, so its products are the dense
+    // path's).  A block row of a few rows would otherwise be a tile of its
+    // own, whose workgroup's fixed chain (x staging, barriers, fold) costs
+    // more than its few hundred values.
+    std::vector<int32_t> brow(std::max<int64_t>(v.nblocks, 1), 0);   // block row of every block
+    for (int32_t a = 0; a < v.nbr; ++a)
+        if (v.bpntrb[a] != -1)
+            for (int64_t k = v.bpntrb[a]; k < v.bpntre[a]; ++k) brow[k] = a;
+    std::vector<int32_t> span_end(v.nbr);
+    {
+        auto rows = [&](int32_t a) { return v.rpntr[a + 1] - v.rpntr[a]; };
+        auto stored = [&](int32_t a) { return v.bpntrb[a] == -1 ? 0 : v.indx[v.bpntre[a]] - v.indx[v.bpntrb[a]]; };
+        const int32_t span_rows = std::min(max_rows, SB_SPMV_SPAN_ROWS);
+        for (int32_t a = 0; a < v.nbr;) {
+            int32_t e = a + 1;
+            auto is_short = [&](int32_t b) { return rows(b) <= SB_SPMV_SHORT_ROWS && stored(b) <= SB_SPMV_SHORT_VALUES; };
+            if (!spmm && is_short(a)) {
+                int64_t nr = rows(a), nv = stored(a);
+                while (e < v.nbr && is_short(e) && nr + rows(e) <= span_rows && nv + stored(e) <= SB_SPMV_SPAN_VALUES) {
+                    nr += rows(e);
+                    nv += stored(e);
+                    ++e;
+                }
+            }
+            span_end[a] = e;
+            a = e;
+        }
+    }
+    std::vector<char> keep;         // dense-routed blocks of a span of several block rows
+    for (int32_t a = 0, a_end; a < v.nbr; a = a_end) {
+        a_end = span_end[a];
+        const bool span = a_end > a + 1;
+        const int32_t r0 = v.rpntr[a], h = v.rpntr[a_end] - r0;
         std::vector<int64_t> dense, sparse, mfma;
-        if (v.bpntrb[a] != -1) {
-            for (int64_t k = v.bpntrb[a]; k < v.bpntre[a]; ++k) {
+        for (int32_t b = a; b < a_end; ++b) {
+            if (v.bpntrb[b] == -1) continue;
+            for (int64_t k = v.bpntrb[b]; k < v.bpntre[b]; ++k) {
                 const uint8_t rt = route ? route[k] : (uint8_t)SABLE_ROUTE_DENSE;
                 if (rt == SABLE_ROUTE_SPARSE) sparse.push_back(k);
                 else if (rt == SABLE_ROUTE_DENSE_MFMA && spmm) {
@@ -422,12 +487,24 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                     }
                     mfma.push_back(k);
                 }
-                else if (rt == SABLE_ROUTE_DENSE || rt == SABLE_ROUTE_DENSE_MFMA) dense.push_back(k);
+                else if (rt == SABLE_ROUTE_DENSE || rt == SABLE_ROUTE_DENSE_MFMA) {
+                    if (span) {
+                        if (keep.empty()) keep.assign(v.nblocks, 0);
+                        keep[k] = 1;
+                        sparse.push_back(k);
+                        S.nblocks_dense += 1;
+                    } else {
+                        dense.push_back(k);
+                    }
+                }
                 else { err = "unknown route value"; return SABLE_ERR_INVALID; }
             }
         }
+        const SpanGeo geo{&v, brow.data(), r0, keep.empty() ? nullptr : &keep};
         S.nblocks_dense += (int64_t)dense.size();
-        S.nblocks_sparse += (int64_t)sparse.size();
+        S.nblocks_sparse += (int64_t)sparse.size() - (span ? (int64_t)std::count_if(sparse.begin(), sparse.end(),
+                                                                                    [&](int64_t k) { return geo.keeps(k); })
+                                                              : 0);
         S.nblocks_mfma += (int64_t)mfma.size();
         // FP32 SpMM pieces stage their X rows in LDS (SB_SPMM_XS_FLOATS)
         const int vcap = spmm ? std::max(1, std::min(o.xcap, SB_SPMM_XS_FLOATS / o.n_rhs)) : o.xcap;
@@ -448,15 +525,13 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
         }
         auto mgroups = group_columns(mfma, v, mcap);
 
-        // sparse nonzeros per row of the block row (for costing / lanes)
-        std::vector<int32_t> row_nnz(h, 0);
+        // sparse entries of the row span (for costing)
         int64_t sp_total = 0;
         for (int64_t k : sparse) {
             const float *bv = v.val + v.indx[k];
-            const int32_t w = v.cpntr[v.bindx[k] + 1] - v.cpntr[v.bindx[k]];
-            for (int32_t c = 0; c < w; ++c)
-                for (int32_t r = 0; r < h; ++r)
-                    if (bv[(int64_t)c * h + r] != 0.0f) { ++row_nnz[r]; ++sp_total; }
+            const int64_t n = (int64_t)(v.cpntr[v.bindx[k] + 1] - v.cpntr[v.bindx[k]]) * geo.height(k);
+            if (geo.keeps(k)) sp_total += n;
+            else for (int64_t i = 0; i < n; ++i) sp_total += (bv[i] != 0.0f);
         }
         int64_t wd = 0, wm = 0;
         for (auto &g : dgroups) wd += group_width(g);
@@ -481,7 +556,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             std::vector<SparseRows> sp;
             for (size_t gi = 0; use_vec && gi < sgroups.size(); ++gi) {
                 SparseRows sr;
-                if (collect_sparse_rows(v, sgroups[gi], h, s0, hs, sr)) {
+                if (collect_sparse_rows(v, sgroups[gi], geo, s0, hs, sr)) {
                     sr.group = (int32_t)gi;
                     sp.push_back(std::move(sr));
                 }

@@ -54,6 +54,15 @@ struct SbTile {
 #define SB_SPUNROLL 2
 #define SB_SPMV_INFLIGHT 1
 
+// SpMV row spans: runs of short block rows (at most SB_SPMV_SHORT_ROWS rows
+// and SB_SPMV_SHORT_VALUES stored values each) are planned as one tile of at
+// most SB_SPMV_SPAN_ROWS rows (and the workgroup size) and
+// SB_SPMV_SPAN_VALUES stored values, every block on the sparse path.
+#define SB_SPMV_SHORT_ROWS 32
+#define SB_SPMV_SHORT_VALUES 8192
+#define SB_SPMV_SPAN_ROWS 256
+#define SB_SPMV_SPAN_VALUES 65536
+
 // LDS for the staged X rows of a FP32 SpMM piece (64 KiB): the plan builder
 // gives such a piece at most SB_SPMM_XS_FLOATS / N columns.
 #define SB_SPMM_XS_FLOATS 16384

@@ -136,6 +136,25 @@ def test_edge_shapes(rt):
     assert np.all(y == 0)
 
 
+@pytest.mark.parametrize("wg", [64, 256, 512])
+def test_short_block_row_spans(rt, wg):
+    """Runs of short block rows planned as one tile each (every block on the
+    sparse path, dense-routed blocks with their zeros): against FP64 on the
+    threshold, all-dense and all-sparse routes; a span with empty block rows
+    and block rows taller than the span limit in between."""
+    vbr, x, _ = synth.mixed_vbr(3000, 600, 1, 12, 0.02, 0.5, (0.8, 1.0), (0.05, 0.3), seed=41)
+    tall, xt, _ = synth.mixed_vbr(2000, 40, 20, 120, 0.1, 0.5, (0.8, 1.0), (0.05, 0.3), seed=42)
+    for name, V, xv in (("short rows", vbr, x), ("short and tall rows", tall, xt)):
+        thr = threshold_routes(V.val, V.indx, 15)
+        for rname, routes in (("threshold", thr), ("dense", np.full_like(thr, ROUTE_DENSE)),
+                              ("sparse", np.full_like(thr, ROUTE_SPARSE))):
+            print(f"case: spans {name} wg={wg} routes={rname}", flush=True)
+            plan = rt.Plan.from_vbr(V, routes, wg=wg)
+            if name == "short rows":
+                assert plan.stats["ntiles"] < (len(V.rpntr) - 1) // 4
+            assert _err(V, xv, plan.spmv(xv))["componentwise"] <= TOL
+
+
 @pytest.mark.parametrize("wg,seg", [(64, 64), (128, 3), (256, 256), (256, 8), (256, 1), (512, 512)])
 def test_sparse_wave_slices(rt, wg, seg):
     """Sparse rows in wave slices: tile heights from 1 row to the workgroup

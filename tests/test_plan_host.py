@@ -227,6 +227,37 @@ def test_host_plan_empty_rows_and_unit_blocks():
     assert np.array_equal(M, D) and np.all(cover == 1)
 
 
+def test_short_block_row_spans():
+    """SpMV: consecutive block rows of at most 32 rows and 8192 stored values
+    form one tile of at most 256 rows (or the workgroup size) and 65536
+    stored values, in which
+    every block takes the sparse path; a dense-routed block keeps its zeros
+    as entries.  The plan reconstructs the matrix exactly, writes every row
+    once, and counts the entries it stores."""
+    vbr, x, _ = synth.mixed_vbr(1500, 300, 1, 10, 0.03, 0.5, (0.8, 1.0), (0.05, 0.3), seed=40)
+    vbr.val[: len(vbr.val) // 3][::7] = 0.0          # zeros inside dense-routed blocks too
+    nbr = len(vbr.rpntr) - 1
+    thr = threshold_routes(vbr.val, vbr.indx, 15).astype(np.uint8)
+    D = vbr_to_dense(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    for wg in (64, 256):
+        plan = Plan.from_vbr(vbr, thr, wg=wg, host_only=True)
+        M, _, cover = reconstruct(plan)
+        assert np.array_equal(M, D) and np.all(cover == 1)
+        tiles = plan.export("tiles")
+        assert len(tiles) < nbr // 4
+        assert all(t["nrows"] <= min(256, wg) for t in tiles)
+        st = plan.stats
+        assert st["dense_values"] == 0
+        sizes = np.diff(np.asarray(vbr.indx))
+        want = sum(int(sizes[k]) if thr[k] == ROUTE_DENSE else
+                   int(np.count_nonzero(vbr.val[vbr.indx[k]:vbr.indx[k + 1]])) for k in range(len(thr)))
+        assert st["sparse_nnz"] == want
+        assert st["nblocks_dense"] == int(np.sum(thr == ROUTE_DENSE))
+    # SpMM plans keep one tile per block row segment
+    plan = Plan.from_vbr(vbr, thr, op="spmm", n_rhs=16, host_only=True)
+    assert len(plan.export("tiles")) >= nbr - int(np.sum(np.asarray(vbr.bpntrb) == -1))
+
+
 def test_host_plan_mfma_layout():
     vbr, x = _small_mixed(11, n=400, parts=6)
     routes = np.where(threshold_routes(vbr.val, vbr.indx, 50) == ROUTE_DENSE, 2, 1).astype(np.uint8)
