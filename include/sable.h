@@ -123,6 +123,9 @@ typedef struct {
     int64_t scratch_bytes;    /* device scratch of the split tiles' partial sums (not
                                  counted in bytes_alg) */
     int64_t fixup_bytes;      /* partial sums written and read back per launch */
+    int64_t nspans;           /* SpMV row spans: tiles of several short block rows,
+                                 every block on the sparse path (0 when they would
+                                 leave the plan below min_tiles) */
 } SablePlanStats;
 
 /* Library. */

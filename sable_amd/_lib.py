@@ -64,7 +64,8 @@ class SablePlanStats(C.Structure):
                 ("rowptr_entries", I64), ("xmap_entries", I64),
                 ("descriptor_bytes", I64), ("bytes_alg", I64), ("flops_alg", I64),
                 ("device_bytes", I64), ("rtc_used", I32), ("mode", I32), ("grid", I32),
-                ("wg", I32), ("ntiles_split", I32), ("scratch_bytes", I64), ("fixup_bytes", I64)]
+                ("wg", I32), ("ntiles_split", I32), ("scratch_bytes", I64), ("fixup_bytes", I64),
+                ("nspans", I64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -372,7 +372,7 @@ double emit_wave_slices(HostPlan &P, SablePlanStats &S, SbTile &t, const std::ve
 }
 
 namespace {
-int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o, HostPlan &P,
+int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o, bool spans, HostPlan &P,
                          std::string &err);
 }
 
@@ -381,16 +381,25 @@ int build_host_plan(const SableVbr &v, const uint8_t *route, const SablePlanOpts
 {
     // Small matrices: halve the tile target until there are min_tiles tiles
     // (or tiles are 16 KiB), so the launch still spreads over the device.
-    SablePlanOpts oo = o;
-    for (;;) {
-        const int rc = build_host_plan_once(v, route, oo, P, err);
-        if (rc || (int64_t)P.tiles.size() >= oo.min_tiles || oo.tile_bytes <= 16 * 1024) return rc;
-        oo.tile_bytes = std::max<int64_t>(16 * 1024, oo.tile_bytes / 2);
+    // Row spans trade tiles for less fixed cost per row; a plan they would
+    // leave below min_tiles is built without them (a small matrix launched
+    // on its own needs the parallelism; plans built for a group take
+    // min_tiles 1 and keep them).
+    for (int spans = 1; spans >= 0; --spans) {
+        SablePlanOpts oo = o;
+        for (;;) {
+            const int rc = build_host_plan_once(v, route, oo, spans != 0, P, err);
+            if (rc) return rc;
+            if ((int64_t)P.tiles.size() >= oo.min_tiles || oo.tile_bytes <= 16 * 1024) break;
+            oo.tile_bytes = std::max<int64_t>(16 * 1024, oo.tile_bytes / 2);
+        }
+        if ((int64_t)P.tiles.size() >= o.min_tiles || !P.stats.nspans) return SABLE_OK;
     }
+    return SABLE_OK;
 }
 
 namespace {
-int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o, HostPlan &P,
+int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePlanOpts &o, bool spans, HostPlan &P,
                          std::string &err)
 {
     int rc = validate_vbr(v, err);
@@ -439,12 +448,14 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
 
     // SpMV: a run of consecutive short block rows (at most SB_SPMV_SHORT_ROWS
     // rows and SB_SPMV_SHORT_VALUES stored values each) is planned as one row
-    // span of at most SB_SPMV_SPAN_ROWS rows and SB_SPMV_SPAN_VALUES stored
-    // values: one tile in which every block takes the sparse path (a
-    // dense-routed block's zeros included, so its products are the dense
-    // path's).  A block row of a few rows would otherwise be a tile of its
-    // own, whose workgroup's fixed chain (x staging, barriers, fold) costs
-    // more than its few hundred values.
+    // span of at most SB_SPMV_SPAN_ROWS rows (and the tile height) and
+    // SB_SPMV_SPAN_VALUES stored values: one tile in which every block takes
+    // the sparse path (a dense-routed block's zeros included, so its
+    // products are the dense path's).  A block row of a few rows would
+    // otherwise be a tile of its own, whose workgroup's fixed chain (x
+    // staging, barriers, fold) costs more than its few hundred values.  (In
+    // SpMM the same spans on the FP32 path measured slower than one MFMA
+    // tile per block row, padding included.)
     std::vector<int32_t> brow(std::max<int64_t>(v.nblocks, 1), 0);   // block row of every block
     for (int32_t a = 0; a < v.nbr; ++a)
         if (v.bpntrb[a] != -1)
@@ -457,7 +468,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
         for (int32_t a = 0; a < v.nbr;) {
             int32_t e = a + 1;
             auto is_short = [&](int32_t b) { return rows(b) <= SB_SPMV_SHORT_ROWS && stored(b) <= SB_SPMV_SHORT_VALUES; };
-            if (!spmm && is_short(a)) {
+            if (spans && !spmm && is_short(a)) {
                 int64_t nr = rows(a), nv = stored(a);
                 while (e < v.nbr && is_short(e) && nr + rows(e) <= span_rows && nv + stored(e) <= SB_SPMV_SPAN_VALUES) {
                     nr += rows(e);
@@ -473,6 +484,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
     for (int32_t a = 0, a_end; a < v.nbr; a = a_end) {
         a_end = span_end[a];
         const bool span = a_end > a + 1;
+        S.nspans += span;
         const int32_t r0 = v.rpntr[a], h = v.rpntr[a_end] - r0;
         std::vector<int64_t> dense, sparse, mfma;
         for (int32_t b = a; b < a_end; ++b) {
@@ -480,6 +492,13 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
             for (int64_t k = v.bpntrb[b]; k < v.bpntre[b]; ++k) {
                 const uint8_t rt = route ? route[k] : (uint8_t)SABLE_ROUTE_DENSE;
                 if (rt == SABLE_ROUTE_SPARSE) sparse.push_back(k);
+                else if (span && (rt == SABLE_ROUTE_DENSE || rt == SABLE_ROUTE_DENSE_MFMA)) {
+                    // a span's dense-routed blocks are entries with their zeros
+                    if (keep.empty()) keep.assign(v.nblocks, 0);
+                    keep[k] = 1;
+                    sparse.push_back(k);
+                    S.nblocks_dense += 1;
+                }
                 else if (rt == SABLE_ROUTE_DENSE_MFMA && spmm) {
                     if (o.n_rhs % 16 != 0 || o.n_rhs > 256) {
                         err = "the MFMA route needs n_rhs a multiple of 16 and at most 256";
@@ -487,16 +506,7 @@ int build_host_plan_once(const SableVbr &v, const uint8_t *route, const SablePla
                     }
                     mfma.push_back(k);
                 }
-                else if (rt == SABLE_ROUTE_DENSE || rt == SABLE_ROUTE_DENSE_MFMA) {
-                    if (span) {
-                        if (keep.empty()) keep.assign(v.nblocks, 0);
-                        keep[k] = 1;
-                        sparse.push_back(k);
-                        S.nblocks_dense += 1;
-                    } else {
-                        dense.push_back(k);
-                    }
-                }
+                else if (rt == SABLE_ROUTE_DENSE || rt == SABLE_ROUTE_DENSE_MFMA) dense.push_back(k);
                 else { err = "unknown route value"; return SABLE_ERR_INVALID; }
             }
         }

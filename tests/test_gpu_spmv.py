@@ -139,7 +139,8 @@ def test_edge_shapes(rt):
 @pytest.mark.parametrize("wg", [64, 256, 512])
 def test_short_block_row_spans(rt, wg):
     """Runs of short block rows planned as one tile each (every block on the
-    sparse path, dense-routed blocks with their zeros): against FP64 on the
+    sparse path, dense-routed blocks with their zeros; min_tiles 1, as for a
+    group member): against FP64 on the
     threshold, all-dense and all-sparse routes; a span with empty block rows
     and block rows taller than the span limit in between."""
     vbr, x, _ = synth.mixed_vbr(3000, 600, 1, 12, 0.02, 0.5, (0.8, 1.0), (0.05, 0.3), seed=41)
@@ -149,7 +150,8 @@ def test_short_block_row_spans(rt, wg):
         for rname, routes in (("threshold", thr), ("dense", np.full_like(thr, ROUTE_DENSE)),
                               ("sparse", np.full_like(thr, ROUTE_SPARSE))):
             print(f"case: spans {name} wg={wg} routes={rname}", flush=True)
-            plan = rt.Plan.from_vbr(V, routes, wg=wg)
+            plan = rt.Plan.from_vbr(V, routes, wg=wg, min_tiles=1)
+            assert plan.stats["nspans"] > 0
             if name == "short rows":
                 assert plan.stats["ntiles"] < (len(V.rpntr) - 1) // 4
             assert _err(V, xv, plan.spmv(xv))["componentwise"] <= TOL

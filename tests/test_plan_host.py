@@ -228,19 +228,26 @@ def test_host_plan_empty_rows_and_unit_blocks():
 
 
 def test_short_block_row_spans():
-    """SpMV: consecutive block rows of at most 32 rows and 8192 stored values
-    form one tile of at most 256 rows (or the workgroup size) and 65536
-    stored values, in which
-    every block takes the sparse path; a dense-routed block keeps its zeros
-    as entries.  The plan reconstructs the matrix exactly, writes every row
-    once, and counts the entries it stores."""
+    """SpMV: consecutive block rows of at most 32 rows and 8192 stored values form
+    one tile of at most 256 rows (or the workgroup size) and 65536 stored
+    values, in which every block takes the sparse path; a dense-routed block
+    keeps its zeros as entries.  Spans are left out when they would leave the
+    plan below min_tiles.  The plan reconstructs the matrix exactly, writes
+    every row once, and counts the entries it stores."""
     vbr, x, _ = synth.mixed_vbr(1500, 300, 1, 10, 0.03, 0.5, (0.8, 1.0), (0.05, 0.3), seed=40)
     vbr.val[: len(vbr.val) // 3][::7] = 0.0          # zeros inside dense-routed blocks too
     nbr = len(vbr.rpntr) - 1
     thr = threshold_routes(vbr.val, vbr.indx, 15).astype(np.uint8)
     D = vbr_to_dense(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr, vbr.bpntrb, vbr.bpntre)
+    # with the default min_tiles (1024) spans would leave this small plan with
+    # too few tiles to spread over the device: it is built without them
+    plan = Plan.from_vbr(vbr, thr, host_only=True)
+    assert plan.stats["nspans"] == 0 and plan.stats["ntiles"] >= nbr - int(np.sum(np.asarray(vbr.bpntrb) == -1))
+    M, _, cover = reconstruct(plan)
+    assert np.array_equal(M, D) and np.all(cover == 1)
     for wg in (64, 256):
-        plan = Plan.from_vbr(vbr, thr, wg=wg, host_only=True)
+        plan = Plan.from_vbr(vbr, thr, wg=wg, min_tiles=1, host_only=True)
+        assert plan.stats["nspans"] > 0
         M, _, cover = reconstruct(plan)
         assert np.array_equal(M, D) and np.all(cover == 1)
         tiles = plan.export("tiles")
@@ -254,7 +261,8 @@ def test_short_block_row_spans():
         assert st["sparse_nnz"] == want
         assert st["nblocks_dense"] == int(np.sum(thr == ROUTE_DENSE))
     # SpMM plans keep one tile per block row segment
-    plan = Plan.from_vbr(vbr, thr, op="spmm", n_rhs=16, host_only=True)
+    plan = Plan.from_vbr(vbr, thr, op="spmm", n_rhs=16, min_tiles=1, host_only=True)
+    assert plan.stats["nspans"] == 0
     assert len(plan.export("tiles")) >= nbr - int(np.sum(np.asarray(vbr.bpntrb) == -1))
 
 
