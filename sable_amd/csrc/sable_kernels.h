@@ -656,12 +656,14 @@ __device__ __forceinline__ void sb_spmm_f32_tile(const SbSpmmArgs &a, int t, int
 }
 
 // Wide shape (SB_F32_WIDE(N)): thread (rg, v) of NV8 = N/8 lanes per row
-// group holds rows [rbeg, rbeg + rt) (rt a multiple of 4) and the column
-// quads 4v and N/2 + 4v, so the lanes of a row group read consecutive
-// float4s of the staged X row in each half.  A dense column's row quads are
-// float4 loads from the padded panel (pitch: the rows rounded up to 4),
-// requested one column pair ahead: two vector-memory loads and four LDS
-// loads per 64 FMAs at rt = 8.
+// group holds rows [rbeg, rbeg + rt) (rt a multiple of 4 in tiles with dense
+// pieces) and the column quads 4v and N/2 + 4v, so the lanes of a row group
+// read consecutive float4s of the staged X row in each half.  A dense
+// column's row quads are float4 loads from the padded panel (pitch: the
+// rows rounded up to 4), requested SB_F32_DEPTH column pairs ahead through a
+// ring of register sets: per column, one vector-memory load and two LDS
+// loads for 32 FMAs at rt = 4.  Only tiles with dense pieces read row quads
+// (nq >= 1 for every active thread there).
 struct SbWideMap {
     int rt;       // rows per row group (multiple of 4, uniform over the tile)
     int rbeg;     // first tile row of this thread's group
