@@ -67,6 +67,11 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 #endif
 // 1: one 16-row unit per wavefront and pass (fewer registers; a taller
 // tile takes more passes over its pieces)
+// 1: four staged k-step buffers and one barrier per two k-steps (where
+// they fit; SB_MFMA_DEPTH even)
+#ifndef SB_MFMA_PAIRS
+#define SB_MFMA_PAIRS 1
+#endif
 #ifndef SB_MFMA_UNITS1
 #define SB_MFMA_UNITS1 0
 #endif
@@ -208,7 +213,20 @@ struct SbMfmaCfg {
     static constexpr int TERM_STRIDE = NMAX * SB_KP;             // bf16 elements per staged term
     static constexpr int UPT = (NMAX * 4 + WG - 1) / WG;         // X staging units per thread
     static constexpr int BUF_STRIDE = SPLIT * TERM_STRIDE;        // one staged k-step (bf16 elements)
-    static constexpr int SMEM_BYTES = 2 * BUF_STRIDE * 2;          // double-buffered
+    // staged k-step buffers: 4 (one barrier per two k-steps) when
+    // SB_MFMA_PAIRS asks for it, a CU holds one workgroup by registers
+    // (SB_MFMA_WAVES waves per SIMD), N is at least 64 and the 4 buffers fit
+    // its LDS (16.5 KiB allowed for xrow); else 2.  At 512 threads and N = 128 they took the
+    // every-block-on-MFMA plan of the 50k SpMM from 120-122 to 116-118 us;
+    // with 256 threads (two workgroups per CU, which 4 buffers would cut to
+    // one) they were 30 % slower, and on the 256-matrix batch's N = 16
+    // members (128 threads, tiles of one or two k-steps) 4.7 % slower.
+    static constexpr int WGS_PER_CU = (4 * SB_MFMA_WAVES) / NW > 0 ? (4 * SB_MFMA_WAVES) / NW : 1;
+    static constexpr int NBUF =
+        SB_MFMA_PAIRS && SB_MFMA_DEPTH % 2 == 0 && WGS_PER_CU == 1 && NMAX >= 64 && 4 * BUF_STRIDE * 2 + 16896 <= 163840
+            ? 4
+            : 2;
+    static constexpr int SMEM_BYTES = NBUF * BUF_STRIDE * 2;
 };
 
 // Fixup of the MFMA row tiles split along k (a launch after the MFMA
@@ -332,22 +350,30 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
     };
 #pragma unroll
     for (int sl = 0; sl < D; ++sl) load_a(ar[sl], sl < K ? sl : K - 1);
-    sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, 0, xf[0]);
-    sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, 0, xf[0], B, Cfg::TERM_STRIDE);
+    // staged k-steps ahead of the one multiplied: 1 with two buffers, 2 with
+    // four (k-step s in buffer s % NBUF; the barrier after every
+    // (NBUF / 2)-th step orders each buffer's writes after its last reads)
+    constexpr int SA = Cfg::NBUF / 2;
+    static_assert(SA <= DX, "the X slots cover the staged k-steps");
 #pragma unroll
-    for (int j = 1; j <= DX; ++j)
+    for (int j = 0; j < SA; ++j) {
+        sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, j < K ? j : K - 1, xf[j % DX]);
+        sb_store_b<WG, Cfg::UPT, SPLIT, NRHS>(N, pc.ncols, j, xf[j % DX], B + j * Cfg::BUF_STRIDE, Cfg::TERM_STRIDE);
+    }
+#pragma unroll
+    for (int j = SA; j < SA + DX; ++j)
         sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, j < K ? j : K - 1, xf[j % DX]);
     split_a(ar[0]);
     load_a(ar[0], D < K ? D : K - 1);
     __syncthreads();
     // k-step ks: `at` holds its A terms; A slot sl (= ks % D) holds the raw
     // A of k-step ks + D, slot (sl + 1) % D that of k-step ks + 1
-    // MASK: the step stages the last k-step, whose columns past the piece
-    // are zeroed (only the last two steps of the piece)
+    // MASK: the step may stage the last k-step, whose columns past the piece
+    // are zeroed (only the last steps of the piece, peeled off the loop)
     auto step = [&](auto slc, auto maskc, const int ks) {
         constexpr int sl = decltype(slc)::value;
         constexpr bool MASK = decltype(maskc)::value != 0;
-        const uint16_t *Bc = B + (ks & 1) * Cfg::BUF_STRIDE;
+        const uint16_t *Bc = B + (ks & (Cfg::NBUF - 1)) * Cfg::BUF_STRIDE;
         if constexpr (NU > 0) {
             using Pt = SbPasses<SPLIT>;
             sb_bf16x8 bt[2][JB][SPLIT];
@@ -382,13 +408,14 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
             __builtin_amdgcn_iglp_opt(SB_MFMA_IGLP);
 #endif
         }
-        // stage k-step ks + 1 into the free buffer (after the last k-step
-        // this writes zeros nobody reads), then load the X values of
-        // k-step ks + 1 + DX into the slot it frees
-        constexpr int xs = (sl + 1) % DX;
-        sb_store_b<WG, Cfg::UPT, SPLIT, NRHS, MASK>(N, pc.ncols, ks + 1, xf[xs],
-                                                    B + ((ks + 1) & 1) * Cfg::BUF_STRIDE, Cfg::TERM_STRIDE);
-        sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + 1 + DX < K ? ks + 1 + DX : K - 1, xf[xs]);
+        // stage k-step ks + SA into its buffer (past the last k-step this
+        // writes zeros nobody reads), then load the X values of k-step
+        // ks + SA + DX into the slot it frees
+        constexpr int xs = (sl + SA) % DX;
+        sb_store_b<WG, Cfg::UPT, SPLIT, NRHS, MASK>(N, pc.ncols, ks + SA, xf[xs],
+                                                    B + ((ks + SA) & (Cfg::NBUF - 1)) * Cfg::BUF_STRIDE,
+                                                    Cfg::TERM_STRIDE);
+        sb_fetch_b<WG, Cfg::UPT, NRHS>(X, N, xrow, pc.ncols, ks + SA + DX < K ? ks + SA + DX : K - 1, xf[xs]);
 #if SB_MFMA_INTERLEAVE
         // issue order of the step: the first column pair's B reads, then
         // every MFMA followed by two VALU (the splits), a B read every
@@ -404,12 +431,12 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
             }
         }
 #endif
-        __syncthreads();
+        if (SA == 1 || (ks & 1)) __syncthreads();
     };
     // main loop: the steps that stage k-steps below K - 1 (every column in
-    // the piece); then the last (at most D + 1) steps with the mask
+    // the piece); then the last (at most D + SA) steps with the mask
     int ks = 0;
-    for (; ks + D + 1 < K; ks += D) {
+    for (; ks + D + SA < K; ks += D) {
         step(SbInt<0>{}, SbInt<0>{}, ks);
         if constexpr (D > 1) step(SbInt<(D > 1 ? 1 : 0)>{}, SbInt<0>{}, ks + 1);
         if constexpr (D > 2) step(SbInt<(D > 2 ? 2 : 0)>{}, SbInt<0>{}, ks + 2);
@@ -420,6 +447,7 @@ __device__ __forceinline__ void sb_mfma_kloop(const SbSpmmArgs &a, const SbMatBu
     if constexpr (D > 2) if (ks + 2 < K) step(SbInt<(D > 2 ? 2 : 0)>{}, SbInt<1>{}, ks + 2);
     if constexpr (D > 3) if (ks + 3 < K) step(SbInt<(D > 3 ? 3 : 0)>{}, SbInt<1>{}, ks + 3);
     if (ks + D < K) step(SbInt<0>{}, SbInt<1>{}, ks + D);
+    if constexpr (SA > 1) if (ks + D + 1 < K) step(SbInt<(D > 1 ? 1 : 0)>{}, SbInt<1>{}, ks + D + 1);
 }
 
 // One pass of a wavefront holding NU units umi[0..NU) over every piece of
