@@ -64,7 +64,8 @@ def _kernel_source(spec):
             KIND_SPMM_MFMA: "sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP, SB_SPLIT>(a);",
             KIND_SPMM_MFMA2: "sb_spmm_mfma_body<SB_WG, SB_NRHS, SB_XCAP, SB_SPLIT>(a);"}[kind]
     args = "SbSpmvArgs" if kind == KIND_SPMV else "SbSpmmArgs"
-    bounds = {KIND_SPMV: "SB_WG, SB_SPMV_WAVES(SB_UNROLL)", KIND_SPMM_F32: "SB_WG"}.get(kind, "SB_WG, SB_MFMA_WAVES")
+    bounds = {KIND_SPMV: "SB_WG, SB_SPMV_WAVES(SB_UNROLL)",
+              KIND_SPMM_F32: "SB_WG, SB_F32_WAVES(SB_NRHS)"}.get(kind, "SB_WG, SB_MFMA_WAVES")
     return (f"// kernel specialisation for this matrix (compiled by hipRTC for gfx950)\n"
             f"#define SB_WG {wg}\n#define SB_UNROLL {unroll}\n#define SB_XCAP {xcap}\n"
             f"#define SB_NRHS {nrhs}\n#define SB_RTMAX {rtmax}\n#define SB_SPLIT {split}\n"
