@@ -70,3 +70,52 @@ def test_random_vbr_sweep(rt, seed):
           flush=True)
     Y = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs, wg=wg).spmm(X)
     assert c_oracle.scaled_error(Y.ravel(), Y64.ravel(), AY.ravel())["componentwise"] <= TOL
+
+
+GROUP_CASES = 6
+
+
+@pytest.mark.parametrize("seed", range(GROUP_CASES))
+def test_random_group_sweep(rt, seed):
+    """Groups of four random members (SpMV or SpMM, random routes, workgroup
+    sizes, RHS widths and MFMA k-split chunks, so one group launches several
+    kernel classes; the first member is an SpMM at N = 128 in 512-thread
+    workgroups, the four-buffer MFMA k-loop): every member's group output is bit-identical to its own
+    launch and within TOL of the FP64 oracle."""
+    g = np.random.default_rng([9, seed])
+    plans, xs, ys, own, ref = [], [], [], [], []
+    for i in range(4):
+        vbr, x = _matrix(100 + 4 * seed + i)
+        nb = len(vbr.bindx)
+        if i > 0 and g.random() < 0.4:
+            wg = int(g.choice([64, 256, 512]))
+            p = rt.Plan.from_vbr(vbr, _routes(g, nb, False), wg=wg, min_tiles=1)
+            inp = np.asarray(x, np.float32)
+            out = p.spmv(inp)
+            y64, ay = c_oracle.spmv_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                     vbr.bpntrb, vbr.bpntre, inp)
+            print(f"case: group seed={seed} member {i} spmv wg={wg}", flush=True)
+        else:
+            n_rhs = 128 if i == 0 else int(g.choice([12, 16, 64, 128]))
+            wg = 512 if i == 0 else int(g.choice([128, 256, 512]))
+            kchunk = int(g.choice([0, 1, 3]))
+            routes = _routes(g, nb, n_rhs % 16 == 0)
+            p = rt.Plan.from_vbr(vbr, routes, op="spmm", n_rhs=n_rhs, wg=wg, mfma_kchunk=kchunk, min_tiles=1)
+            inp = g.uniform(-1, 1, (int(vbr.cpntr[-1]), n_rhs)).astype(np.float32)
+            out = p.spmm(inp)
+            y64, ay = c_oracle.spmm_f64_dense_blocks(vbr.val, vbr.indx, vbr.bindx, vbr.rpntr, vbr.cpntr,
+                                                     vbr.bpntrb, vbr.bpntre, inp)
+            print(f"case: group seed={seed} member {i} spmm n_rhs={n_rhs} wg={wg} kchunk={kchunk} "
+                  f"mfma={int(np.sum(routes == ROUTE_DENSE_MFMA))}", flush=True)
+        plans.append(p)
+        xs.append(rt.DeviceArray.from_numpy(inp))
+        ys.append(rt.DeviceArray(out.shape))
+        own.append(out)
+        ref.append((y64, ay))
+    grp = rt.Group(plans, xs, ys)
+    grp.launch()
+    for y, o, (y64, ay) in zip(ys, own, ref):
+        got = y.numpy()
+        assert np.array_equal(got, o)
+        assert c_oracle.scaled_error(got.ravel(), np.ravel(y64), np.ravel(ay))["componentwise"] <= TOL
+    grp.close()
