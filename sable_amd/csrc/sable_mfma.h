@@ -50,7 +50,8 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 // the compiler's scheduling strategy for a k-step's basic block
 // (__builtin_amdgcn_iglp_opt), -1: none.  Strategy 3 ran the 50k SpMM at
 // N = 128 6 % faster than none (116-117 vs 124 us, bit-identical) and the
-// 256-matrix batch at N = 16 the same
+// 256-matrix batch at N = 16 the same; with four staged buffers, 114-115
+// against 120-121 us without it
 #ifndef SB_MFMA_IGLP
 #define SB_MFMA_IGLP 3
 #endif
@@ -65,13 +66,13 @@ typedef __attribute__((ext_vector_type(4))) float sb_f32x4;
 #ifndef SB_MFMA_PACK
 #define SB_MFMA_PACK 0
 #endif
-// 1: one 16-row unit per wavefront and pass (fewer registers; a taller
-// tile takes more passes over its pieces)
 // 1: four staged k-step buffers and one barrier per two k-steps (where
 // they fit; SB_MFMA_DEPTH even)
 #ifndef SB_MFMA_PAIRS
 #define SB_MFMA_PAIRS 1
 #endif
+// 1: one 16-row unit per wavefront and pass (fewer registers; a taller
+// tile takes more passes over its pieces)
 #ifndef SB_MFMA_UNITS1
 #define SB_MFMA_UNITS1 0
 #endif
@@ -214,16 +215,19 @@ struct SbMfmaCfg {
     static constexpr int UPT = (NMAX * 4 + WG - 1) / WG;         // X staging units per thread
     static constexpr int BUF_STRIDE = SPLIT * TERM_STRIDE;        // one staged k-step (bf16 elements)
     // staged k-step buffers: 4 (one barrier per two k-steps) when
-    // SB_MFMA_PAIRS asks for it, a CU holds one workgroup by registers
+    // SB_MFMA_PAIRS asks for it (with A and X prefetch depths of at least
+    // two), a CU holds one workgroup by registers
     // (SB_MFMA_WAVES waves per SIMD), N is at least 64 and the 4 buffers fit
     // its LDS (16.5 KiB allowed for xrow); else 2.  At 512 threads and N = 128 they took the
-    // every-block-on-MFMA plan of the 50k SpMM from 120-122 to 116-118 us;
+    // every-block-on-MFMA plan of the 50k SpMM from 120-122 to 116-118 us
+    // (118-119 to 114-115 us in a run timing two copies of each);
     // with 256 threads (two workgroups per CU, which 4 buffers would cut to
     // one) they were 30 % slower, and on the 256-matrix batch's N = 16
     // members (128 threads, tiles of one or two k-steps) 4.7 % slower.
     static constexpr int WGS_PER_CU = (4 * SB_MFMA_WAVES) / NW > 0 ? (4 * SB_MFMA_WAVES) / NW : 1;
     static constexpr int NBUF =
-        SB_MFMA_PAIRS && SB_MFMA_DEPTH % 2 == 0 && WGS_PER_CU == 1 && NMAX >= 64 && 4 * BUF_STRIDE * 2 + 16896 <= 163840
+        SB_MFMA_PAIRS && SB_MFMA_DEPTH % 2 == 0 && SB_MFMA_XDEPTH >= 2 && WGS_PER_CU == 1 && NMAX >= 64 &&
+                4 * BUF_STRIDE * 2 + 16896 <= 163840
             ? 4
             : 2;
     static constexpr int SMEM_BYTES = NBUF * BUF_STRIDE * 2;
