@@ -141,6 +141,21 @@ def retime(plan, x, y, args, flush):
     return plan.time(x, y, warmup=args.warmup, reps=args.steps, flush_bytes=flush)
 
 
+SPINUP_S = 0.15
+
+
+def retime_spun_up(plan, x, y, args, flush):
+    """The same measurement after SPINUP_S seconds of the plan's own untimed
+    launches.  After the host-side work between variants the GPU idles, and
+    the first ~70 timed launches of the 50k SpMM then read 3-15 % slower
+    than the steady ones; this figure is reported beside the standard one,
+    never in its place."""
+    t0 = time.time()
+    while time.time() - t0 < SPINUP_S:
+        plan.time(x, y, warmup=0, reps=10, flush_bytes=flush)
+    return plan.time(x, y, warmup=args.warmup, reps=args.steps, flush_bytes=flush)
+
+
 def spmm_c3(args, runtime, vbr, X, base, cpu=None):
     """SpMM on the 50,000 x 50,000 VBR (500 x 500 partitions sized 32-192, 2 %
     blocks, 70 % density 0.9-1.0 and 30 % 0.02-0.1, N(0,1); X 50,000 x 128
@@ -220,6 +235,7 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
         t = retime(plans[name], Xd, Yd, args, flush)
         res[name]["retimed_ms"] = float(np.mean(t))
         res[name]["retimed_median_ms"] = float(np.median(t))
+        res[name]["spun_up_ms"] = float(np.mean(retime_spun_up(plans[name], Xd, Yd, args, flush)))
         if name == kept:
             assert c_oracle.scaled_error(Yd.numpy().ravel(), Y64.ravel(), AY.ravel())["componentwise"] <= 1e-5
     for p in plans.values():
@@ -255,6 +271,9 @@ def spmm_c3(args, runtime, vbr, X, base, cpu=None):
         "kept": kept, "ms": ms, "gflops": gflops, "error_vs_fp64": best["err"], "roofline": roof,
         "roofline_times_us": {"mfma_6pass": t_mfma * 1e6, "hbm": t_hbm * 1e6},
         "speedup_vs_fp32_plain": fp32_ms / ms if plain else None, "fp32_plain_ms": fp32_ms,
+        "spun_up": {"note": f"the same timing after {SPINUP_S:g} s of the plan's own untimed launches "
+                            "(GPU clocks out of their idle state); reported beside ms, never in its place",
+                    "ms": best.get("spun_up_ms"), "fp32_plain_ms": plain.get("spun_up_ms") if plain else None},
         "fp32_plain_roofline": fp32_roof,
         "speedup_x3_vs_fp32_plain": plain["ms"] / res["mfma_x3"]["ms"] if plain and "mfma_x3" in res else None,
         "variants": {k: {"ms": v["ms"], "median_ms": v["median_ms"], "err": v["err"]["componentwise"],
@@ -856,6 +875,7 @@ def main():
     errs = c_oracle.scaled_error(yd.numpy(), y64, ay)
     assert errs["componentwise"] <= 1e-5, errs
     warm = plans[label].time(xd, yd, warmup=3, reps=max(5, args.steps // 2), flush_bytes=0)
+    spun = retime_spun_up(plans[label], xd, yd, args, flush)
     for p in plans.values():
         p.close()
     ms = float(np.mean(times))
@@ -908,6 +928,7 @@ def main():
         "kept": kept, "kept_plan": label,
         "median_ms": float(np.median(times)), "p10_ms": float(np.percentile(times, 10)),
         "p90_ms": float(np.percentile(times, 90)), "warm_ms": float(np.mean(warm)),
+        "spun_up_ms": float(np.mean(spun)),
         "error_vs_fp64": errs, "bytes_alg": head["bytes_alg"],
         "routings": res,
         "learned_speedup_vs_plain": (res["plain"]["ms"] / res["learned"]["ms"]) if "learned" in res and "plain" in res
