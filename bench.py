@@ -420,6 +420,12 @@ def bench_batch(args, runtime, info):
     g_plain.close()
     t_learned_unbatched = runtime.time_plans(learned, xs, ys_l, warmup=1, reps=max(3, args.steps // 2),
                                              flush_bytes=flush)
+    # the kept group again after SPINUP_S seconds of its own untimed replays
+    # (reported beside ms_per_step, never in its place)
+    t0 = time.time()
+    while time.time() - t0 < SPINUP_S:
+        g.time(warmup=0, reps=3, flush_bytes=flush)
+    t_batch_spun = g.time(warmup=2, reps=args.steps, flush_bytes=flush)
     g.launch()
     errs = sample_errors(ys_l)
     g.close()
@@ -455,6 +461,7 @@ def bench_batch(args, runtime, info):
         "unbatched_learned_ms": float(np.mean(t_learned_unbatched)),
         "batched_learned_ms_by_bias": {str(bb): float(np.mean(t)) for bb, t in zip(biases, t_cands)},
         "speedup_vs_unbatched_plain": ms_plain / ms,
+        "spun_up_ms": float(np.mean(t_batch_spun)),
         "bytes_alg_learned": bytes_learned, "bytes_alg_plain": bytes_plain, "flops_alg": flops,
         "sample_errors_vs_fp64": {"max": max(errs), "checked_products": len(errs)},
         "sample_errors_by_bias": {str(bb): e for bb, e in zip(biases, e_cands)},
